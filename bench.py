@@ -1,0 +1,165 @@
+"""AO rays/s on the production workload (2048^2 map over a 2.1M-triangle
+heightfield, AO at 256 samples), one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-bake]
+
+A step is one AO-256 pass over every texel of the map (one launch of the
+hemisphere kernel, ~1.07e9 rays), with the scene, texel set, mapping result
+and sample table already resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "heightfield2048"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (see DESIGN.md "Measurement"):
+TEXEL_BYTES = 16 + 4 + 96 + 4  # coords + tidx in, hit-triangle positions+normals, AO out
+RAY_BYTES = 16                 # one sample direction
+NODE_BYTES = 32                # one node test
+TRI_BYTES = 64                 # one triangle test (predicate record)
+
+
+def load_traffic():
+    """HBM bytes per AO launch from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", "ao_heightfield2048_pmc.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(prep, mm, solver, budget_s=12.0):
+    """The oracle (C restatement of the shaders, OpenMP over texels) on a
+    bounded sample of the same AO-256 workload: contiguous 256-texel chunks
+    spread over the map until ~budget_s of wall time."""
+    import numpy as np
+    import oracle as orc
+    nodes, _ = prep["scene"].reference_nodes()
+    spos, snrm = prep["scene"].soup()
+    coords, tidx = mm.read()
+    p = prep["cfg"]["maps"]["ao"]
+    S = p["sampleCount"]
+    n = len(tidx)
+    threads = os.cpu_count() or 1
+    starts = np.linspace(0, n - 256, 64).astype(int)
+    order = np.concatenate([starts[::8], np.delete(starts, np.arange(0, 64, 8))])
+    rays, texels, t = 0, 0, 0.0
+    for s in order:
+        t0 = time.perf_counter()
+        orc.hemisphere(orc.KIND_AO, S, p["minDistance"], p["maxDistance"], solver.table, coords, tidx,
+                       spos, snrm, nodes, int(s), int(s) + 256, nthreads=threads)
+        t += time.perf_counter() - t0
+        live = int(np.sum(tidx[s:s + 256] != 0xFFFFFFFF))
+        rays += live * S
+        texels += 256
+        if t > budget_s:
+            break
+    return {"value": rays / t, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": "%d texels (%d-texel chunks spread over the map) x %d AO samples = %d rays, %.1f s"
+                      % (texels, 256, S, rays, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bake", action="store_true")
+    ap.add_argument("--workload", default=WORKLOAD)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import fornos_amd as fa
+
+    if fa.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    prep = fa.prepare(args.workload)
+    prep_s = time.perf_counter() - t0
+    cfg = prep["cfg"]
+    scene_info = prep["scene"].info()
+    print("# prepared %s in %.1f s: %d hi tris, %d nodes, %d texels"
+          % (args.workload, prep_s, scene_info["n_triangles"], scene_info["n_nodes"], prep["texels"].count),
+          file=sys.stderr, flush=True)
+
+    mm = fa.MeshMapping().init(prep["texels"], prep["scene"], cfg["cull"])
+    map_ms = mm.run_all()
+    p = cfg["maps"]["ao"]
+    ao = fa.AmbientOcclusionSolver(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(prep["texels"], mm)
+    n = mm.count
+    _, tidx = mm.read()
+    live = int(np.sum(tidx != 0xFFFFFFFF))
+    S = p["sampleCount"]
+    rays_per_step = live * S
+
+    for _ in range(args.warmup):
+        ao.reset()
+        ao.run_all()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ao.reset()
+        kernel_ms.append(ao.run_all())  # ends with hipEventSynchronize on the launch stream
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    value = rays_per_step * args.steps / wall
+    print("# AO-%d: %.2f ms/step, %.3e rays/s" % (S, ms_per_step, value), file=sys.stderr, flush=True)
+
+    work = ao.count_work()
+    k_ms = float(np.mean(kernel_ms))
+    alg_bytes = (n * TEXEL_BYTES + work["rays"] * RAY_BYTES + work["node_tests"] * NODE_BYTES
+                 + work["tri_tests"] * TRI_BYTES)
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                "kernel": "k_hemi<AO>", "kernel_ms": k_ms, "algorithmic_bytes": alg_bytes,
+                "node_tests_per_ray": work["node_tests"] / max(1, work["rays"]),
+                "tri_tests_per_ray": work["tri_tests"] / max(1, work["rays"])}
+
+    bake = None
+    if not args.no_bake:
+        # seconds per full 2048^2 bake: mapping + every map the workload names
+        t0 = time.perf_counter()
+        mm2 = fa.MeshMapping().init(prep["texels"], prep["scene"], cfg["cull"])
+        parts = {"mapping": mm2.run_all()}
+        for name, kw in cfg["maps"].items():
+            cls = {"height": fa.HeightSolver, "position": fa.PositionSolver, "normals": fa.NormalsSolver,
+                   "ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
+                   "thickness": fa.ThicknessSolver}[name]
+            s = cls(**kw).init(prep["texels"], mm2)
+            parts[name] = s.run_all()
+            s.getResults()
+        bake = {"seconds": time.perf_counter() - t0, "kernel_ms": parts}
+
+    cpu = None if args.no_cpu else cpu_baseline(prep, mm, ao)
+    line = {
+        "metric": "AO rays/s", "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural meshes)",
+        "config": {"workload": "%s: 2048^2 map, low 64x64-quad grid, high 1024x1024-quad fBm+bumps "
+                               "heightfield (%d tris), AO %d samples min %.2f max %.2f"
+                               % (args.workload, scene_info["n_triangles"], S, p["minDistance"], p["maxDistance"]),
+                   "texels": n, "rays_per_step": rays_per_step, "bvh": "reference binary, 32 B nodes"},
+        "roofline": roofline, "cpu_baseline": cpu, "mapping_ms": map_ms, "bake": bake,
+        "texels_per_s": live / (ms_per_step * 1e-3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""ctypes binding of libfornos_hip.so (include/fornos_hip.h, include/fornos_host.h).
+
+The library is built in-tree by fornos_amd.build; loading fails loudly when it
+is missing, so no code path silently runs without the native library.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfornos_hip.so")
+
+OK, INVALID_ARG, OOM, HIP_ERROR, UNSUPPORTED = 0, 1, 2, 3, 4
+
+GEN_UV_SPHERE, GEN_ICOSPHERE_WAVES, GEN_HEIGHTFIELD, GEN_HEIGHTFIELD_LOW, GEN_TORUS_KNOT = range(5)
+SOLVER_HEIGHT, SOLVER_POSITION, SOLVER_NORMALS, SOLVER_AO, SOLVER_BENT_NORMALS, SOLVER_THICKNESS = range(6)
+BVH_REFERENCE = 0
+
+
+class MeshView(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("n_positions", C.c_uint32),
+                ("normals", C.c_void_p), ("n_normals", C.c_uint32),
+                ("texcoords", C.c_void_p), ("n_texcoords", C.c_uint32),
+                ("tangents", C.c_void_p), ("bitangents", C.c_void_p),
+                ("vertices", C.c_void_p), ("n_vertices", C.c_uint32),
+                ("triangles", C.c_void_p), ("n_triangles", C.c_uint32)]
+
+
+class BVHParams(C.Structure):
+    _fields_ = [("layout", C.c_int), ("max_tris_per_leaf", C.c_uint32), ("max_depth", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("n_triangles", C.c_uint32), ("n_leaves", C.c_uint32),
+                ("max_leaf_tris", C.c_uint32), ("depth", C.c_uint32),
+                ("build_seconds", C.c_double), ("device_bytes", C.c_size_t)]
+
+
+class TexelView(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("positions", C.c_void_p), ("directions", C.c_void_p), ("normals", C.c_void_p),
+                ("tangents", C.c_void_p), ("bitangents", C.c_void_p), ("indices", C.c_void_p)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("sample_count", C.c_uint32), ("min_distance", C.c_float),
+                ("max_distance", C.c_float), ("tangent_space", C.c_int),
+                ("normalize_output", C.c_int)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("a", C.c_float), ("b", C.c_uint32), ("c", C.c_uint32)]
+
+
+H = C.c_void_p
+P = C.c_void_p
+u32p = C.POINTER(C.c_uint32)
+
+# (name, restype, argtypes); restype None means an int status.
+_SIGS = [
+    ("fornos_last_error", C.c_char_p, []),
+    ("fornos_version", C.c_char_p, []),
+    ("fornos_device_count", C.c_int, []),
+    # host
+    ("fornos_mesh_generate", None, [C.c_int, C.c_uint64, C.POINTER(GenParams), C.POINTER(H)]),
+    ("fornos_mesh_from_view", None, [C.POINTER(MeshView), C.POINTER(H)]),
+    ("fornos_mesh_copy", None, [H, C.POINTER(H)]),
+    ("fornos_mesh_view_get", None, [H, C.POINTER(MeshView)]),
+    ("fornos_mesh_compute_vertex_normals", None, [H]),
+    ("fornos_mesh_compute_vertex_normals_aggressive", None, [H]),
+    ("fornos_mesh_compute_tangent_space", None, [H]),
+    ("fornos_mesh_destroy", "void", [H]),
+    ("fornos_texels_create", None, [H, H, C.c_uint32, C.c_uint32, C.POINTER(H)]),
+    ("fornos_texels_view_get", None, [H, C.POINTER(TexelView)]),
+    ("fornos_texels_export_mapuv", None, [H, P, P, P, P, P]),
+    ("fornos_texels_destroy", "void", [H]),
+    ("fornos_bvh_reference_export", None, [H, C.c_uint32, C.c_uint32, u32p, P, P]),
+    ("fornos_sample_table", None, [C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
+    # device
+    ("fornos_scene_create", None, [C.POINTER(MeshView), C.POINTER(BVHParams), C.POINTER(H)]),
+    ("fornos_scene_info_get", None, [H, C.POINTER(SceneInfo)]),
+    ("fornos_scene_export_reference_nodes", None, [H, P, P]),
+    ("fornos_scene_export_soup", None, [H, P, P]),
+    ("fornos_scene_destroy", "void", [H]),
+    ("fornos_meshmap_create", None, [H, C.POINTER(TexelView), C.c_int, C.POINTER(H)]),
+    ("fornos_meshmap_run_step", None, [H, C.POINTER(C.c_int)]),
+    ("fornos_meshmap_run_all", None, [H, C.POINTER(C.c_float)]),
+    ("fornos_meshmap_progress", None, [H, C.POINTER(C.c_float)]),
+    ("fornos_meshmap_reset", None, [H]),
+    ("fornos_meshmap_read", None, [H, P, P]),
+    ("fornos_meshmap_texel_count", None, [H, u32p]),
+    ("fornos_meshmap_destroy", "void", [H]),
+    ("fornos_solver_create", None, [C.c_int, C.POINTER(SolverParams), H, P, C.c_uint32, C.POINTER(H)]),
+    ("fornos_solver_run_step", None, [H, C.POINTER(C.c_int)]),
+    ("fornos_solver_run_all", None, [H, C.POINTER(C.c_float)]),
+    ("fornos_solver_progress", None, [H, C.POINTER(C.c_float)]),
+    ("fornos_solver_reset", None, [H]),
+    ("fornos_solver_read", None, [H, P, C.c_size_t]),
+    ("fornos_solver_read_counts", None, [H, P]),
+    ("fornos_solver_count_work", None, [H, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("fornos_solver_destroy", "void", [H]),
+]
+
+SYMBOLS = [s[0] for s in _SIGS]
+
+
+class FornosError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("fornos status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads the native library (once); raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FornosError(UNSUPPORTED, "%s is missing: run `python -m fornos_amd.build` first" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in _SIGS:
+        f = getattr(L, name)
+        f.argtypes = args
+        if res is None:
+            f.restype = C.c_int
+        elif res == "void":
+            f.restype = None
+        else:
+            f.restype = res
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != OK:
+        raise FornosError(status, lib().fornos_last_error().decode(errors="replace"))
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def view_array(addr, count, dtype, shape):
+    """Copies `count` elements at a native address into a new numpy array."""
+    if not addr or count == 0:
+        return np.zeros(shape, dtype=dtype)
+    n = int(np.prod(shape))
+    buf = (C.c_byte * (n * np.dtype(dtype).itemsize)).from_address(addr)
+    return np.frombuffer(buf, dtype=dtype, count=n).reshape(shape).copy()

@@ -1,0 +1,325 @@
+"""Python mirror of the reference's baking interface, over the C ABI.
+
+Names follow the reference classes: MeshMapping.init/runStep/progress
+(Src/meshmapping.h:42-57) and the solvers' init/runStep/getResults/progress
+(Src/solver_*.h).  Arrays cross as numpy; device memory is owned by the
+native handles.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib, ptr, view_array
+
+SAMPLE_PERM_COUNT = 64 * 64  # k_samplePermCount, Src/solver_ao.cpp:34
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self._h = C.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h and self._h.value:
+            getattr(lib(), self._destroy)(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh(_Handle):
+    """The reference Mesh (Src/mesh.h:36-77), held by the native library."""
+    _destroy = "fornos_mesh_destroy"
+
+    @classmethod
+    def generate(cls, generator, seed=0, a=0.0, b=0, c=0):
+        m = cls()
+        gp = _abi.GenParams(float(a), int(b), int(c))
+        check(lib().fornos_mesh_generate(int(generator), int(seed), C.byref(gp), C.byref(m._h)))
+        return m
+
+    @classmethod
+    def from_arrays(cls, positions, vertices, triangles, normals=None, texcoords=None):
+        m = cls()
+        keep = [np.ascontiguousarray(x, dtype=t) if x is not None else None for x, t in
+                ((positions, np.float32), (vertices, np.uint32), (triangles, np.uint32),
+                 (normals, np.float32), (texcoords, np.float32))]
+        p, v, t, n, uv = keep
+        mv = _abi.MeshView(ptr(p), len(p), ptr(n), 0 if n is None else len(n), ptr(uv),
+                           0 if uv is None else len(uv), None, None, ptr(v), len(v), ptr(t), len(t))
+        check(lib().fornos_mesh_from_view(C.byref(mv), C.byref(m._h)))
+        return m
+
+    def copy(self):
+        m = Mesh()
+        check(lib().fornos_mesh_copy(self._h, C.byref(m._h)))
+        return m
+
+    def mesh_view(self):
+        mv = _abi.MeshView()
+        check(lib().fornos_mesh_view_get(self._h, C.byref(mv)))
+        return mv
+
+    def arrays(self):
+        """Copies of the mesh arrays as a dict of numpy arrays."""
+        v = self.mesh_view()
+        out = {
+            "positions": view_array(v.positions, v.n_positions * 3, np.float32, (v.n_positions, 3)),
+            "normals": view_array(v.normals, v.n_normals * 3, np.float32, (v.n_normals, 3)),
+            "texcoords": view_array(v.texcoords, v.n_texcoords * 2, np.float32, (v.n_texcoords, 2)),
+            "vertices": view_array(v.vertices, v.n_vertices * 3, np.uint32, (v.n_vertices, 3)),
+            "triangles": view_array(v.triangles, v.n_triangles * 3, np.uint32, (v.n_triangles, 3)),
+        }
+        if v.tangents:
+            out["tangents"] = view_array(v.tangents, v.n_vertices * 3, np.float32, (v.n_vertices, 3))
+            out["bitangents"] = view_array(v.bitangents, v.n_vertices * 3, np.float32, (v.n_vertices, 3))
+        return out
+
+    @property
+    def n_triangles(self):
+        return self.mesh_view().n_triangles
+
+    def computeVertexNormals(self):
+        check(lib().fornos_mesh_compute_vertex_normals(self._h))
+
+    def computeVertexNormalsAggressive(self):
+        check(lib().fornos_mesh_compute_vertex_normals_aggressive(self._h))
+
+    def computeTangentSpace(self):
+        check(lib().fornos_mesh_compute_tangent_space(self._h))
+
+
+class CompressedMapUV(_Handle):
+    """MapUV + CompressedMapUV (Src/compute.h:214-259) of a low-poly mesh."""
+    _destroy = "fornos_texels_destroy"
+
+    def __init__(self, lo, dirs, width, height):
+        super().__init__()
+        check(lib().fornos_texels_create(lo.handle, dirs.handle if dirs is not None else None,
+                                         int(width), int(height), C.byref(self._h)))
+        self.width, self.height = int(width), int(height)
+
+    def texel_view(self):
+        tv = _abi.TexelView()
+        check(lib().fornos_texels_view_get(self._h, C.byref(tv)))
+        return tv
+
+    @property
+    def count(self):
+        return self.texel_view().count
+
+    def arrays(self):
+        tv = self.texel_view()
+        n = tv.count
+        f = lambda a: view_array(a, n * 3, np.float32, (n, 3))
+        return {"positions": f(tv.positions), "directions": f(tv.directions), "normals": f(tv.normals),
+                "tangents": f(tv.tangents), "bitangents": f(tv.bitangents),
+                "indices": view_array(tv.indices, n, np.uint32, (n,))}
+
+    def mapuv(self):
+        npx = self.width * self.height
+        outs = [np.empty((npx, 3), np.float32) for _ in range(5)]
+        check(lib().fornos_texels_export_mapuv(self._h, *[ptr(o) for o in outs]))
+        return dict(zip(["positions", "directions", "normals", "tangents", "bitangents"], outs))
+
+
+def sample_table(sample_count, n_perm=SAMPLE_PERM_COUNT, seed=1234):
+    """(table[n_perm*S, 3] float32, rand[n_perm, 2] uint32), Src/math.h:363-382."""
+    t = np.empty((n_perm * sample_count, 3), np.float32)
+    r = np.empty((n_perm, 2), np.uint32)
+    check(lib().fornos_sample_table(int(sample_count), int(n_perm), int(seed), ptr(t), ptr(r)))
+    return t, r
+
+
+class Scene(_Handle):
+    """Hi-poly mesh + BVH resident in HBM (BVH::createBinary + fillMeshData)."""
+    _destroy = "fornos_scene_destroy"
+
+    def __init__(self, mesh, bvh_tris_per_node=8, max_depth=8192):
+        super().__init__()
+        self.mesh = mesh
+        bp = _abi.BVHParams(_abi.BVH_REFERENCE, int(bvh_tris_per_node), int(max_depth))
+        mv = mesh.mesh_view()
+        check(lib().fornos_scene_create(C.byref(mv), C.byref(bp), C.byref(self._h)))
+
+    def info(self):
+        i = _abi.SceneInfo()
+        check(lib().fornos_scene_info_get(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def reference_nodes(self):
+        """(nodes[n,9] uint32 in BVHGPUData layout, mesh triangle id per soup triangle)."""
+        i = self.info()
+        nodes = np.empty((i["n_nodes"], 9), np.uint32)
+        ids = np.empty(i["n_triangles"], np.uint32)
+        check(lib().fornos_scene_export_reference_nodes(self._h, ptr(nodes), ptr(ids)))
+        return nodes, ids
+
+    def soup(self):
+        i = self.info()
+        p = np.empty((3 * i["n_triangles"], 4), np.float32)
+        n = np.empty((3 * i["n_triangles"], 4), np.float32)
+        check(lib().fornos_scene_export_soup(self._h, ptr(p), ptr(n)))
+        return p, n
+
+
+class MeshMapping(_Handle):
+    """MeshMapping (Src/meshmapping.h:42-75)."""
+    _destroy = "fornos_meshmap_destroy"
+
+    def init(self, texels, scene, cullBackfaces=False):
+        self.close()
+        self.texels, self.scene = texels, scene
+        tv = texels.texel_view()
+        check(lib().fornos_meshmap_create(scene.handle, C.byref(tv), int(bool(cullBackfaces)), C.byref(self._h)))
+        return self
+
+    def runStep(self):
+        done = C.c_int()
+        check(lib().fornos_meshmap_run_step(self._h, C.byref(done)))
+        return bool(done.value)
+
+    def run_all(self):
+        ms = C.c_float()
+        check(lib().fornos_meshmap_run_all(self._h, C.byref(ms)))
+        return ms.value
+
+    def progress(self):
+        p = C.c_float()
+        check(lib().fornos_meshmap_progress(self._h, C.byref(p)))
+        return p.value
+
+    def reset(self):
+        check(lib().fornos_meshmap_reset(self._h))
+
+    @property
+    def count(self):
+        n = C.c_uint32()
+        check(lib().fornos_meshmap_texel_count(self._h, C.byref(n)))
+        return n.value
+
+    def read(self):
+        """(coords[n,4] = t, b0, b1, b2; tidx[n] soup vertex index or 0xFFFFFFFF)."""
+        n = self.count
+        coords = np.empty((n, 4), np.float32)
+        tidx = np.empty(n, np.uint32)
+        check(lib().fornos_meshmap_read(self._h, ptr(coords), ptr(tidx)))
+        return coords, tidx
+
+
+class Solver(_Handle):
+    """One of the reference solvers (Src/solver_*.h), selected by kind."""
+    _destroy = "fornos_solver_destroy"
+    kind = None
+    comps = 1
+
+    def __init__(self, sample_count=0, min_distance=0.0, max_distance=0.0, tangent_space=False,
+                 normalize_output=False, seed=1234, table=None):
+        super().__init__()
+        self.params = _abi.SolverParams(int(sample_count), float(min_distance), float(max_distance),
+                                        int(bool(tangent_space)), int(bool(normalize_output)))
+        self.seed = seed
+        self.table = table
+
+    def init(self, texels, meshMapping):
+        self.close()
+        self.meshMapping = meshMapping
+        tab, n_perm = None, 0
+        if self.kind >= _abi.SOLVER_AO:
+            if self.table is None:
+                self.table, _ = sample_table(self.params.sample_count, SAMPLE_PERM_COUNT, self.seed)
+            tab = np.ascontiguousarray(self.table, np.float32)
+            n_perm = len(tab) // self.params.sample_count
+        check(lib().fornos_solver_create(self.kind, C.byref(self.params), meshMapping.handle,
+                                         ptr(tab), n_perm, C.byref(self._h)))
+        return self
+
+    def runStep(self):
+        done = C.c_int()
+        check(lib().fornos_solver_run_step(self._h, C.byref(done)))
+        return bool(done.value)
+
+    def run_all(self):
+        ms = C.c_float()
+        check(lib().fornos_solver_run_all(self._h, C.byref(ms)))
+        return ms.value
+
+    def progress(self):
+        p = C.c_float()
+        check(lib().fornos_solver_progress(self._h, C.byref(p)))
+        return p.value
+
+    def reset(self):
+        check(lib().fornos_solver_reset(self._h))
+
+    def getResults(self):
+        n = self.meshMapping.count
+        shape = (n,) if self.comps == 1 else (n, self.comps)
+        out = np.empty(shape, np.float32)
+        check(lib().fornos_solver_read(self._h, ptr(out), out.nbytes))
+        return out
+
+    def counts(self):
+        out = np.empty(self.meshMapping.count, np.uint32)
+        check(lib().fornos_solver_read_counts(self._h, ptr(out)))
+        return out
+
+    def count_work(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"node_tests": a.value, "tri_tests": b.value, "rays": c.value}
+
+
+class HeightSolver(Solver):
+    kind = _abi.SOLVER_HEIGHT
+
+    def __init__(self, normalizeOutput=True, maxDistance=0.0):
+        super().__init__(max_distance=maxDistance, normalize_output=normalizeOutput)
+
+
+class PositionSolver(Solver):
+    kind, comps = _abi.SOLVER_POSITION, 3
+
+
+class NormalsSolver(Solver):
+    kind, comps = _abi.SOLVER_NORMALS, 3
+
+    def __init__(self, tangentSpace=True):
+        super().__init__(tangent_space=tangentSpace)
+
+
+class AmbientOcclusionSolver(Solver):
+    kind = _abi.SOLVER_AO
+
+    def __init__(self, sampleCount=256, minDistance=0.01, maxDistance=10.0, seed=1234, table=None):
+        super().__init__(sampleCount, minDistance, maxDistance, seed=seed, table=table)
+
+
+class BentNormalsSolver(Solver):
+    kind, comps = _abi.SOLVER_BENT_NORMALS, 3
+
+    def __init__(self, sampleCount=256, minDistance=0.01, maxDistance=10.0, tangentSpace=True,
+                 seed=1234, table=None):
+        super().__init__(sampleCount, minDistance, maxDistance, tangent_space=tangentSpace,
+                         seed=seed, table=table)
+
+
+class ThicknessSolver(Solver):
+    kind = _abi.SOLVER_THICKNESS
+
+    def __init__(self, sampleCount=256, minDistance=0.01, maxDistance=10.0, seed=1234, table=None):
+        super().__init__(sampleCount, minDistance, maxDistance, seed=seed, table=table)
+
+
+def device_count():
+    return lib().fornos_device_count()

@@ -1,0 +1,280 @@
+// bvh.cpp -- the reference's binary BVH (Src/bvh.cpp:63-314) built on the
+// host and flattened in its DFS order with skip links
+// (Src/meshmapping.cpp:62-125) into 32-byte device nodes.
+//
+// The tree is the reference's, node for node: 16 centroid buckets per axis,
+// its surface-area cost (with the xy term counted twice and yz missing,
+// Src/bvh.cpp:66), split at a bucket boundary, stable partition by centroid,
+// leaf at <= max_tris or depth >= max_depth or when a side comes out empty,
+// node boxes stored as centre/half-size and read back as centre -/+ size.
+// Unlike the reference it keeps one index array partitioned in place (stable,
+// through a scratch array) and builds disjoint subtrees on separate threads;
+// the result does not depend on the thread count.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <memory>
+
+#include "host.h"
+
+namespace fornos {
+namespace {
+
+struct Box {
+    V3 minv = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+    V3 maxv = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    void add(V3 p) { minv = hmin(p, minv); maxv = hmax(p, maxv); }  // BucketAABB::addPoint
+};
+// combine(aabb0, aabb1), Src/bvh.cpp:55-61
+Box combine(const Box& a0, const Box& a1)
+{
+    Box r = a0;
+    r.minv = hmin(a1.minv, r.minv);
+    r.maxv = hmax(a1.maxv, r.maxv);
+    return r;
+}
+// computeCost, Src/bvh.cpp:63-68
+float cost(const Box& b, int count)
+{
+    V3 s = b.maxv - b.minv;
+    float area = 2.0f * (s.x * s.y + s.x * s.z + s.x * s.y);
+    return (float)count * area;
+}
+
+struct Node {
+    V3 center, size;
+    size_t subtree = 0;
+    std::unique_ptr<Node[]> child;  // null or 2
+    uint32_t begin = 0, end = 0;    // leaf triangle range in the index array
+};
+
+struct Builder {
+    const fornos_mesh& m;
+    size_t max_tris, max_depth;
+    std::vector<V3> p0, p1, p2;     // triangle corners
+    std::vector<uint32_t> idx, scratch;
+
+    Builder(const fornos_mesh& mesh, size_t mt, size_t md) : m(mesh), max_tris(mt), max_depth(md)
+    {
+        const uint32_t nt = m.ntri();
+        p0.resize(nt); p1.resize(nt); p2.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            p0[t] = m.pos_of(m.triangles[3 * t]);
+            p1[t] = m.pos_of(m.triangles[3 * t + 1]);
+            p2[t] = m.pos_of(m.triangles[3 * t + 2]);
+        }
+        idx.resize(nt);
+        scratch.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t) idx[t] = t;
+    }
+
+    // selectSplitFromBuckets, Src/bvh.cpp:70-126.  At i == 15 the reference
+    // reads one past aabbRightSide[15]; that cost term is multiplied by
+    // countRight == 0 and contributes 0.
+    static void select(const uint32_t bk[16], const Box bb[16], size_t count, int& best_idx, float& best_cost)
+    {
+        Box right[15];
+        right[14] = bb[15];
+        for (int i = 13; i >= 0; --i) right[i] = combine(right[i + 1], bb[i + 1]);
+        best_cost = FLT_MAX;
+        best_idx = 0;
+        Box left;
+        int cl = 0, cr = (int)count;
+        for (int i = 0; i < 16; ++i) {
+            cl += (int)bk[i];
+            cr -= (int)bk[i];
+            left = combine(left, bb[i]);
+            float c = cost(left, cl) + (i < 15 ? cost(right[i], cr) : 0.0f);
+            if (c < best_cost) { best_cost = c; best_idx = i; }
+        }
+    }
+
+    // findBestSplit, Src/bvh.cpp:128-203
+    void find_split(uint32_t b, uint32_t e, int& axis, float& split) const
+    {
+        Box cb;
+        for (uint32_t k = b; k < e; ++k) {
+            uint32_t t = idx[k];
+            cb.add((p0[t] + p1[t] + p2[t]) / 3.0f);
+        }
+        uint32_t bx[16] = {0}, by[16] = {0}, bz[16] = {0};
+        Box ax[16], ay[16], az[16];
+        const V3 ext = cb.maxv - cb.minv;
+        for (uint32_t k = b; k < e; ++k) {
+            uint32_t t = idx[k];
+            V3 c = (p0[t] + p1[t] + p2[t]) / 3.0f;
+            V3 d = c - cb.minv;
+            V3 ijk = mk(d.x / ext.x * 15.99f, d.y / ext.y * 15.99f, d.z / ext.z * 15.99f);
+            size_t i = std::isnan(ijk.x) ? 0 : (size_t)ijk.x;
+            size_t j = std::isnan(ijk.y) ? 0 : (size_t)ijk.y;
+            size_t l = std::isnan(ijk.z) ? 0 : (size_t)ijk.z;
+            ++bx[i]; ++by[j]; ++bz[l];
+            ax[i].add(p0[t]); ax[i].add(p1[t]); ax[i].add(p2[t]);
+            ay[j].add(p0[t]); ay[j].add(p1[t]); ay[j].add(p2[t]);
+            az[l].add(p0[t]); az[l].add(p1[t]); az[l].add(p2[t]);
+        }
+        int ix, iy, iz;
+        float cx, cy, cz;
+        select(bx, ax, e - b, ix, cx);
+        select(by, ay, e - b, iy, cy);
+        select(bz, az, e - b, iz, cz);
+        if (cx <= cy && cx <= cz) {
+            axis = 0; split = cb.minv.x + (cb.maxv.x - cb.minv.x) / 16.0f * (float)(ix + 1);
+        } else if (cy <= cx && cy <= cz) {
+            axis = 1; split = cb.minv.y + (cb.maxv.y - cb.minv.y) / 16.0f * (float)(iy + 1);
+        } else {
+            axis = 2; split = cb.minv.z + (cb.maxv.z - cb.minv.z) / 16.0f * (float)(iz + 1);
+        }
+    }
+
+    // binaryDivisionBVH, Src/bvh.cpp:205-281
+    void divide(Node& par, size_t depth)
+    {
+        const uint32_t b = par.begin, e = par.end, n = e - b;
+        if (n <= max_tris || depth >= max_depth) { par.subtree = n; return; }
+        int axis;
+        float split;
+        find_split(b, e, axis, split);
+        V3 minL = mk(FLT_MAX, FLT_MAX, FLT_MAX), maxL = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        V3 minR = minL, maxR = maxL;
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t k = b; k < e; ++k) {
+            uint32_t t = idx[k];
+            V3 c = (p0[t] + p1[t] + p2[t]) * (1.0f / 3.0f);
+            float cv = axis == 0 ? c.x : (axis == 1 ? c.y : c.z);
+            if (cv <= split) {
+                idx[b + nl++] = t;  // stable: writes never overtake reads
+                minL = hmin(minL, hmin(p0[t], hmin(p1[t], p2[t])));
+                maxL = hmax(maxL, hmax(p0[t], hmax(p1[t], p2[t])));
+            } else {
+                scratch[b + nr++] = t;
+                minR = hmin(minR, hmin(p0[t], hmin(p1[t], p2[t])));
+                maxR = hmax(maxR, hmax(p0[t], hmax(p1[t], p2[t])));
+            }
+        }
+        std::memcpy(&idx[b + nl], &scratch[b], sizeof(uint32_t) * nr);
+        if (nl == 0 || nr == 0) { par.subtree = n; return; }  // "Unable to subdivide"
+        par.child.reset(new Node[2]);
+        Node& L = par.child[0];
+        Node& R = par.child[1];
+        L.begin = b; L.end = b + nl;
+        R.begin = b + nl; R.end = e;
+        L.center = (minL + maxL) * 0.5f; L.size = (maxL - minL) * 0.5f;
+        R.center = (minR + maxR) * 0.5f; R.size = (maxR - minR) * 0.5f;
+        if (n > 65536 && depth < 6) {
+            auto f = std::async(std::launch::async, [&] { divide(L, depth + 1); });
+            divide(R, depth + 1);
+            f.get();
+        } else {
+            divide(L, depth + 1);
+            divide(R, depth + 1);
+        }
+        V3 aminL = L.center - L.size, amaxL = L.center + L.size;
+        V3 aminR = R.center - R.size, amaxR = R.center + R.size;
+        V3 amin = hmin(aminL, aminR), amax = hmax(amaxL, amaxR);
+        par.center = (amax + amin) * 0.5f;
+        par.size = (amax - amin) * 0.5f;
+        par.begin = par.end = 0;
+        par.subtree = L.subtree + R.subtree;
+    }
+};
+
+struct Flattener {
+    const Builder& B;
+    FlatBVH& out;
+    uint32_t soup = 0;  // soup triangles emitted so far
+
+    void put_f(size_t w, float f) { std::memcpy(&out.nodes[w], &f, 4); }
+
+    // fillMeshData, Src/meshmapping.cpp:62-125
+    void run(const Node& n, uint32_t depth)
+    {
+        const bool leaf = !n.child;
+        if (leaf && n.end == n.begin) return;
+        if (!leaf) {
+            if (n.child[0].subtree > 0 && n.child[1].subtree == 0) { run(n.child[0], depth); return; }
+            if (n.child[1].subtree > 0 && n.child[0].subtree == 0) { run(n.child[1], depth); return; }
+        }
+        const uint32_t index = out.n_nodes++;
+        out.nodes.resize((size_t)out.n_nodes * 8);
+        const V3 mn = n.center - n.size, mx = n.center + n.size;
+        const size_t w = (size_t)index * 8;
+        put_f(w + 0, mn.x); put_f(w + 1, mn.y); put_f(w + 2, mn.z);
+        out.nodes[w + 3] = soup;
+        put_f(w + 4, mx.x); put_f(w + 5, mx.y); put_f(w + 6, mx.z);
+        out.depth = std::max(out.depth, depth);
+        if (leaf) {
+            const uint32_t cnt = n.end - n.begin;
+            for (uint32_t k = n.begin; k < n.end; ++k) out.soup_tri[soup++] = B.idx[k];
+            out.nodes[w + 7] = kLeafFlag | cnt;
+            out.n_leaves++;
+            out.max_leaf = std::max(out.max_leaf, cnt);
+            return;
+        }
+        run(n.child[0], depth + 1);
+        run(n.child[1], depth + 1);
+        out.nodes[(size_t)index * 8 + 7] = out.n_nodes;
+    }
+};
+
+}  // namespace
+
+void build_reference_bvh(const fornos_mesh& m, uint32_t max_tris, uint32_t max_depth, FlatBVH& out)
+{
+    Builder B(m, max_tris, max_depth);
+    // BVH::createBinary: root box over every mesh position (Src/bvh.cpp:288-300)
+    V3 mins = mk(FLT_MAX, FLT_MAX, FLT_MAX), maxs = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    for (const V3& p : m.positions) { mins = hmin(mins, p); maxs = hmax(maxs, p); }
+    Node root;
+    root.center = (maxs + mins) * 0.5f;
+    root.size = (maxs - mins) * 0.5f;
+    root.begin = 0;
+    root.end = m.ntri();
+    B.divide(root, 0);
+    out = FlatBVH();
+    out.soup_tri.resize(m.ntri());
+    out.nodes.reserve((size_t)16 * m.ntri() / std::max<uint32_t>(1, max_tris) + 16);
+    Flattener F{B, out};
+    F.run(root, 0);
+    out.soup_tri.resize(F.soup);
+}
+
+void export_reference_nodes(const FlatBVH& B, uint32_t* nodes9)
+{
+    for (uint32_t i = 0; i < B.n_nodes; ++i) {
+        const uint32_t* w = &B.nodes[8 * (size_t)i];
+        uint32_t* o = &nodes9[9 * (size_t)i];
+        o[0] = w[0]; o[1] = w[1]; o[2] = w[2]; o[3] = w[4]; o[4] = w[5]; o[5] = w[6];
+        if (w[7] & kLeafFlag) {
+            o[6] = 3 * w[3];
+            o[7] = 3 * (w[3] + (w[7] & ~kLeafFlag));
+            o[8] = i + 1;
+        } else {
+            o[6] = o[7] = 3 * w[3];
+            o[8] = w[7];
+        }
+    }
+}
+
+}  // namespace fornos
+
+extern "C" fornos_status fornos_bvh_reference_export(const fornos_mesh* m, uint32_t max_tris,
+                                                     uint32_t max_depth, uint32_t* n_nodes,
+                                                     uint32_t* nodes9, uint32_t* ids)
+{
+    if (!m || !n_nodes || max_tris == 0)
+        return fornos::set_error(FORNOS_INVALID_ARG, "fornos_bvh_reference_export: bad argument");
+    fornos::FlatBVH B;
+    fornos::build_reference_bvh(*m, max_tris, max_depth, B);
+    if (nodes9) {
+        if (*n_nodes < B.n_nodes)
+            return fornos::set_error(FORNOS_INVALID_ARG, "fornos_bvh_reference_export: node buffer too small");
+        fornos::export_reference_nodes(B, nodes9);
+        if (ids) std::memcpy(ids, B.soup_tri.data(), sizeof(uint32_t) * B.soup_tri.size());
+    }
+    *n_nodes = B.n_nodes;
+    return FORNOS_OK;
+}

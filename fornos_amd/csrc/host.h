@@ -1,0 +1,61 @@
+// host.h -- internal host-side types of libfornos_hip (not part of the ABI).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fornos_host.h"
+#include "../../include/fornos_hip.h"
+#include "vec.h"
+
+// The reference's Mesh (Src/mesh.h:36-77): shared attribute arrays plus one
+// {position, texcoord, normal} index triple per vertex and three vertex
+// indices per triangle.
+struct fornos_mesh {
+    std::vector<fornos::V3> positions;
+    std::vector<fornos::V2> texcoords;
+    std::vector<fornos::V3> normals;
+    std::vector<fornos::V3> tangents;    // per vertex (may be empty)
+    std::vector<fornos::V3> bitangents;  // per vertex (may be empty)
+    std::vector<uint32_t> vertices;      // 3 per vertex
+    std::vector<uint32_t> triangles;     // 3 per triangle
+
+    uint32_t nv() const { return (uint32_t)(vertices.size() / 3); }
+    uint32_t ntri() const { return (uint32_t)(triangles.size() / 3); }
+    fornos::V3 pos_of(uint32_t vtx) const { return positions[vertices[3 * vtx]]; }
+};
+
+// MapUV (Src/compute.h:214-241) and CompressedMapUV (:245-259) together.
+struct fornos_texels {
+    uint32_t width = 0, height = 0;
+    std::vector<float> map_pos, map_dir, map_nrm, map_tan, map_bit;  // W*H*3 each
+    std::vector<float> pos, dir, nrm, tan, bit;                      // count*3 each
+    std::vector<uint32_t> indices;
+};
+
+namespace fornos {
+
+fornos_status set_error(fornos_status st, const std::string& msg);
+fornos_status ok();
+
+// mesh.cpp
+void vertex_normals(fornos_mesh& m);             // Src/mesh.cpp:650-678
+void vertex_normals_aggressive(fornos_mesh& m);  // Src/mesh.cpp:680-721
+void tangent_space(fornos_mesh& m);              // Src/mesh.cpp:724-768
+
+// texels.cpp
+bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                  fornos_texels& out);
+
+// Reference-layout BVH, flattened (bvh.cpp).
+struct FlatBVH {
+    // 8 words per node: min.xyz, first soup triangle, max.xyz, jump | leaf flag + count
+    std::vector<uint32_t> nodes;
+    std::vector<uint32_t> soup_tri;  // mesh triangle id per soup triangle
+    uint32_t n_nodes = 0, n_leaves = 0, max_leaf = 0, depth = 0;
+};
+constexpr uint32_t kLeafFlag = 0x80000000u;
+void build_reference_bvh(const fornos_mesh& m, uint32_t max_tris, uint32_t max_depth, FlatBVH& out);
+void export_reference_nodes(const FlatBVH& B, uint32_t* nodes9);  // BVHGPUData, 9 words per node
+
+}  // namespace fornos

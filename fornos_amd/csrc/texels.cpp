@@ -1,0 +1,214 @@
+// texels.cpp -- the texel set: MapUV raster (Src/compute.cpp:75-193) and
+// CompressedMapUV (Src/compute.cpp:372-416), the sample table
+// (Src/math.h:324-382) and the library's error state.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "host.h"
+
+using namespace fornos;
+
+namespace fornos {
+
+static thread_local std::string g_last_error;
+
+fornos_status set_error(fornos_status st, const std::string& msg)
+{
+    g_last_error = msg;
+    return st;
+}
+fornos_status ok() { return FORNOS_OK; }
+
+namespace {
+
+// Src/math.h:250-258
+V3 bary2(V2 p, V2 p0, V2 p1, V2 p2)
+{
+    V2 v0 = {p1.x - p0.x, p1.y - p0.y}, v1 = {p2.x - p0.x, p2.y - p0.y}, v2 = {p.x - p0.x, p.y - p0.y};
+    float s = 1.0f / (v0.x * v1.y - v1.x * v0.y);
+    float i = (v2.x * v1.y - v1.x * v2.y) * s;
+    float j = (v0.x * v2.y - v2.x * v0.y) * s;
+    return mk(1.0f - i - j, i, j);
+}
+
+// uint32_t(std::roundf(x)) as the reference's x86-64 build evaluates it: a
+// 64-bit truncating conversion of which the low 32 bits are kept.
+uint32_t round_u32(float x) { return (uint32_t)(int64_t)std::roundf(x); }
+
+V3 blend(V3 a, V3 b, V3 c, V3 w) { return a * w.x + b * w.y + c * w.z; }
+
+}  // namespace
+
+bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                  fornos_texels& out)
+{
+    const size_t npx = (size_t)W * H;
+    out.width = W;
+    out.height = H;
+    out.map_pos.assign(3 * npx, 0.0f);
+    out.map_dir.assign(3 * npx, 0.0f);
+    out.map_nrm.assign(3 * npx, 0.0f);
+    out.map_tan.assign(3 * npx, 0.0f);
+    out.map_bit.assign(3 * npx, 0.0f);
+    const V2 scale = {(float)W, (float)H};
+    const V2 pixsize = {1.0f / scale.x, 1.0f / scale.y};
+    const V2 halfpix = {pixsize.x * 0.5f, pixsize.y * 0.5f};
+    const bool has_tan = !lo.tangents.empty();
+    const uint32_t nt = lo.ntri();
+    for (uint32_t t = 0; t < nt; ++t) {
+        uint32_t vi[3] = {lo.triangles[3 * t], lo.triangles[3 * t + 1], lo.triangles[3 * t + 2]};
+        for (int k = 0; k < 3; ++k)
+            if (lo.vertices[3 * vi[k] + 1] == UINT32_MAX || lo.vertices[3 * vi[k] + 2] == UINT32_MAX)
+                return false;
+        V3 p[3], n[3], d[3], tg[3], bt[3];
+        V2 u[3], s[3];
+        for (int k = 0; k < 3; ++k) {
+            p[k] = lo.positions[lo.vertices[3 * vi[k]]];
+            u[k] = lo.texcoords[lo.vertices[3 * vi[k] + 1]];
+            n[k] = lo.normals[lo.vertices[3 * vi[k] + 2]];
+            d[k] = dirs ? dirs->normals[dirs->vertices[3 * vi[k] + 2]] : n[k];
+            tg[k] = has_tan ? lo.tangents[vi[k]] : mk(0, 0, 0);
+            bt[k] = has_tan ? lo.bitangents[vi[k]] : mk(0, 0, 0);
+            s[k] = {(u[k].x - halfpix.x) * scale.x, (u[k].y - halfpix.y) * scale.y};
+        }
+        const uint32_t xMin = std::min(round_u32(fminf(s[0].x, fminf(s[1].x, s[2].x))), W);
+        const uint32_t yMin = std::min(round_u32(fminf(s[0].y, fminf(s[1].y, s[2].y))), H);
+        const uint32_t xMax = std::min(round_u32(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x))), W);
+        const uint32_t yMax = std::min(round_u32(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y))), H);
+        for (uint32_t y = yMin; y <= yMax; ++y)
+            for (uint32_t x = xMin; x <= xMax; ++x) {
+                V2 q = {(float)x * pixsize.x + halfpix.x, (float)y * pixsize.y + halfpix.y};
+                V3 b = bary2(q, u[0], u[1], u[2]);
+                if (b.x >= -0.001f && b.x <= 1 && b.y >= -0.001f && b.y <= 1 && b.z >= -0.001f && b.z <= 1) {
+                    const size_t i = (size_t)y * W + x;
+                    if (i >= npx) continue;  // the reference writes past the end here (y == H)
+                    st3(out.map_pos.data(), i, blend(p[0], p[1], p[2], b));
+                    st3(out.map_dir.data(), i, normalize(blend(d[0], d[1], d[2], b)));
+                    st3(out.map_nrm.data(), i, normalize(blend(n[0], n[1], n[2], b)));
+                    st3(out.map_tan.data(), i, normalize(blend(tg[0], tg[1], tg[2], b)));
+                    st3(out.map_bit.data(), i, normalize(blend(bt[0], bt[1], bt[2], b)));
+                }
+            }
+    }
+    // CompressedMapUV: texels with dot(dir, dir) > 0.5, row-major.
+    out.indices.clear();
+    for (size_t i = 0; i < npx; ++i) {
+        V3 n = ld3(out.map_dir.data(), i);
+        if (dot(n, n) > 0.5f) out.indices.push_back((uint32_t)i);
+    }
+    const size_t c = out.indices.size();
+    out.pos.resize(3 * c); out.dir.resize(3 * c); out.nrm.resize(3 * c);
+    out.tan.resize(3 * c); out.bit.resize(3 * c);
+    for (size_t k = 0; k < c; ++k) {
+        const size_t i = out.indices[k];
+        std::memcpy(&out.pos[3 * k], &out.map_pos[3 * i], 12);
+        std::memcpy(&out.dir[3 * k], &out.map_dir[3 * i], 12);
+        std::memcpy(&out.nrm[3 * k], &out.map_nrm[3 * i], 12);
+        std::memcpy(&out.tan[3 * k], &out.map_tan[3 * i], 12);
+        std::memcpy(&out.bit[3 * k], &out.map_bit[3 * i], 12);
+    }
+    return true;
+}
+
+}  // namespace fornos
+
+namespace {
+
+// Src/math.h:324-332
+uint32_t reverse_bits(uint32_t bits)
+{
+    bits = (bits << 16u) | (bits >> 16u);
+    bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+    bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+    bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+    bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+    return bits;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fornos_last_error(void) { return g_last_error.c_str(); }
+
+fornos_status fornos_texels_create(const fornos_mesh* lo, const fornos_mesh* dirs, uint32_t W,
+                                   uint32_t H, fornos_texels** out)
+{
+    if (!lo || !out) return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: null argument");
+    if (W == 0 || H == 0 || W > 16384 || H > 16384)
+        return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: map size must be in [1, 16384]");
+    if (dirs && (dirs->nv() != lo->nv()))
+        return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: mapping mesh must be a copy of the low-poly mesh");
+    *out = nullptr;
+    fornos_texels* t = new (std::nothrow) fornos_texels();
+    if (!t) return set_error(FORNOS_OOM, "fornos_texels_create: out of host memory");
+    try {
+        if (!build_texels(*lo, dirs, W, H, *t)) {
+            delete t;
+            return set_error(FORNOS_INVALID_ARG, "Low poly mesh is missing texture coordinates or normals information");
+        }
+    } catch (const std::bad_alloc&) {
+        delete t;
+        return set_error(FORNOS_OOM, "fornos_texels_create: out of host memory");
+    }
+    *out = t;
+    return ok();
+}
+
+fornos_status fornos_texels_view_get(const fornos_texels* t, fornos_texel_view* v)
+{
+    if (!t || !v) return set_error(FORNOS_INVALID_ARG, "fornos_texels_view_get: null argument");
+    v->count = (uint32_t)t->indices.size();
+    v->width = t->width;
+    v->height = t->height;
+    v->positions = t->pos.data();
+    v->directions = t->dir.data();
+    v->normals = t->nrm.data();
+    v->tangents = t->tan.data();
+    v->bitangents = t->bit.data();
+    v->indices = t->indices.data();
+    return ok();
+}
+
+fornos_status fornos_texels_export_mapuv(const fornos_texels* t, float* p, float* d, float* n,
+                                         float* tg, float* b)
+{
+    if (!t) return set_error(FORNOS_INVALID_ARG, "fornos_texels_export_mapuv: null texels");
+    const size_t bytes = t->map_pos.size() * sizeof(float);
+    if (p) std::memcpy(p, t->map_pos.data(), bytes);
+    if (d) std::memcpy(d, t->map_dir.data(), bytes);
+    if (n) std::memcpy(n, t->map_nrm.data(), bytes);
+    if (tg) std::memcpy(tg, t->map_tan.data(), bytes);
+    if (b) std::memcpy(b, t->map_bit.data(), bytes);
+    return ok();
+}
+
+void fornos_texels_destroy(fornos_texels* t) { delete t; }
+
+// Src/math.h:339-344 (hammersley with Cranley-Patterson shift on x and XOR
+// scrambling on y) and :363-382 (cosine-weighted hemisphere directions).
+fornos_status fornos_sample_table(uint32_t S, uint32_t P, uint32_t seed, float* out, uint32_t* rand_out)
+{
+    if (!out || S == 0 || P == 0) return set_error(FORNOS_INVALID_ARG, "fornos_sample_table: bad argument");
+    std::mt19937 re(seed);
+    for (uint32_t j = 0; j < P; ++j) {
+        const uint32_t rx = (uint32_t)re(), ry = (uint32_t)re();
+        if (rand_out) { rand_out[2 * j] = rx; rand_out[2 * j + 1] = ry; }
+        for (uint32_t i = 0; i < S; ++i) {
+            const float x = (float)i / (float)S + (float)(rx & 0xffffu) / (float)(1 << 16);
+            const float y = (float)(reverse_bits(i) ^ ry) * 2.3283064365386963e-10f;
+            const float ux = x - (float)(long)x;
+            const float r = std::sqrt(ux);
+            const float phi = (float)(2.0 * 3.14159265) * y;
+            float* o = &out[3 * ((size_t)j * S + i)];
+            o[0] = r * std::cos(phi);
+            o[1] = r * std::sin(phi);
+            o[2] = std::sqrt(1.0f - ux);
+        }
+    }
+    return ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+/*
+ * fornos_hip.h -- C ABI of the MI355X baking hot path.
+ *
+ * Plain C: POD structs, pointers and sizes, opaque handles, an int status on
+ * every call.  No STL, GL or torch type crosses it.  Each entry point
+ * replaces one method of the reference's GL-backed classes; the reference
+ * interface it stands in for is cited (file:line, relative to the reference
+ * root).  Host-side input producers (meshes, texel sets, sample tables) are
+ * declared in fornos_host.h.
+ *
+ * Threading: a handle's work is queued on one HIP stream owned by its scene;
+ * calls on one handle (and on handles sharing a scene) must be serialised by
+ * the caller.  Distinct scenes may be driven from distinct threads.
+ */
+#ifndef FORNOS_HIP_H
+#define FORNOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+typedef int fornos_status;
+#define FORNOS_OK 0
+#define FORNOS_INVALID_ARG 1
+#define FORNOS_OOM 2
+#define FORNOS_HIP_ERROR 3
+#define FORNOS_UNSUPPORTED 4
+
+/* Thread-local message describing the last failed call on this thread.
+ * Replaces the reference's logError strings (Src/logging.cpp:43-74). */
+const char *fornos_last_error(void);
+
+/* Library version string. */
+const char *fornos_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int fornos_device_count(void);
+
+/* ---- mesh view (the hi-poly Mesh of Src/mesh.h:36-77) ------------------- */
+/* vertices: 3 uint32 per vertex {positionIndex, texcoordIndex, normalIndex}
+ * (Mesh::Vertex, Src/mesh.h:38-43; UINT32_MAX = absent); triangles: 3 vertex
+ * indices each (Mesh::Triangle, Src/mesh.h:45-50). */
+typedef struct fornos_mesh_view {
+    const float *positions;  /* 3 * n_positions */
+    uint32_t n_positions;
+    const float *normals;    /* 3 * n_normals */
+    uint32_t n_normals;
+    const float *texcoords;  /* 2 * n_texcoords (may be NULL) */
+    uint32_t n_texcoords;
+    const float *tangents;   /* 3 * n_vertices (may be NULL) */
+    const float *bitangents; /* 3 * n_vertices (may be NULL) */
+    const uint32_t *vertices;
+    uint32_t n_vertices;
+    const uint32_t *triangles;
+    uint32_t n_triangles;
+} fornos_mesh_view;
+
+/* ---- scene: hi-poly mesh + BVH resident in HBM ------------------------- */
+/* BVH layouts.  FORNOS_BVH_REFERENCE is the reference's binary tree
+ * (Src/bvh.cpp:283-314, leaf <= max_tris_per_leaf or depth >= max_depth)
+ * flattened in its DFS order with skip links (Src/meshmapping.cpp:62-125),
+ * stored as 32-byte nodes. */
+#define FORNOS_BVH_REFERENCE 0
+
+typedef struct fornos_bvh_params {
+    int layout;                 /* FORNOS_BVH_* */
+    uint32_t max_tris_per_leaf; /* FornosParameters_Shared::bvhTrisPerNode, default 8 (Src/fornos.h:44) */
+    uint32_t max_depth;         /* 8192 in the reference (Src/fornos.cpp:157) */
+} fornos_bvh_params;
+
+typedef struct fornos_scene_info {
+    uint32_t n_nodes;
+    uint32_t n_triangles;   /* soup triangles (= mesh triangles) */
+    uint32_t n_leaves;
+    uint32_t max_leaf_tris;
+    uint32_t depth;
+    double build_seconds;   /* host BVH build + flatten */
+    size_t device_bytes;    /* HBM held by the scene */
+} fornos_scene_info;
+
+typedef struct fornos_scene fornos_scene;
+
+/* BVH::createBinary + fillMeshData + the GL uploads of MeshMapping::init
+ * (Src/bvh.cpp:283, Src/meshmapping.cpp:128-162). */
+fornos_status fornos_scene_create(const fornos_mesh_view *hi, const fornos_bvh_params *params,
+                                  fornos_scene **out);
+fornos_status fornos_scene_info_get(const fornos_scene *scene, fornos_scene_info *out);
+/* Exports the node array in the reference's BVHGPUData layout
+ * (Src/meshmapping.h:33-40: 6 floats min/max, start, end, jump; 9 words per
+ * node; start/end are soup vertex indices), and the mesh triangle id of each
+ * soup triangle (the flatten permutation).  Either pointer may be NULL. */
+fornos_status fornos_scene_export_reference_nodes(const fornos_scene *scene, uint32_t *nodes9,
+                                                  uint32_t *soup_triangle_ids);
+/* Soup positions / normals as the reference's Vector4 arrays (4 floats per
+ * soup vertex, Src/meshmapping.cpp:99-116).  Either pointer may be NULL. */
+fornos_status fornos_scene_export_soup(const fornos_scene *scene, float *positions4,
+                                       float *normals4);
+void fornos_scene_destroy(fornos_scene *scene);
+
+/* ---- mesh mapping (MeshMapping, Src/meshmapping.h:42-75) --------------- */
+/* texels: CompressedMapUV arrays (Src/compute.h:245-259), 3 floats per texel.
+ * normals/tangents/bitangents may be NULL when no tangent-space pass runs. */
+typedef struct fornos_texel_view {
+    uint32_t count;
+    uint32_t width, height;
+    const float *positions;
+    const float *directions;
+    const float *normals;
+    const float *tangents;
+    const float *bitangents;
+    const uint32_t *indices;    /* MapUV index of each texel (may be NULL) */
+} fornos_texel_view;
+
+typedef struct fornos_meshmap fornos_meshmap;
+
+/* MeshMapping::init(map, mesh, bvh, cullBackfaces) (Src/meshmapping.cpp:128-184). */
+fornos_status fornos_meshmap_create(fornos_scene *scene, const fornos_texel_view *texels,
+                                    int cull_backfaces, fornos_meshmap **out);
+/* MeshMapping::runStep (Src/meshmapping.cpp:186-218): one chunk of at most
+ * 131072 texels; *done = 1 once every texel is mapped. */
+fornos_status fornos_meshmap_run_step(fornos_meshmap *mm, int *done);
+/* Every remaining texel in one launch.  *ms (may be NULL) receives the
+ * kernel time measured with HIP events on the scene's stream. */
+fornos_status fornos_meshmap_run_all(fornos_meshmap *mm, float *ms);
+/* Rewinds progress to 0 so the next run_step/run_all maps every texel again
+ * (the reference re-runs a pass only through a new init). */
+fornos_status fornos_meshmap_reset(fornos_meshmap *mm);
+/* MeshMapping::progress (Src/meshmapping.h:48). */
+fornos_status fornos_meshmap_progress(const fornos_meshmap *mm, float *out);
+/* Readback of r_coords (t, b0, b1, b2 per texel) and r_tidx (soup vertex
+ * index of the hit triangle, 0xFFFFFFFF on a miss), the contents of the
+ * buffers MeshMapping::coords()/coords_tidx() expose (Src/meshmapping.h:50-51).
+ * Either pointer may be NULL; count entries each. */
+fornos_status fornos_meshmap_read(const fornos_meshmap *mm, float *coords4, uint32_t *tidx);
+/* Number of texels (CompressedMapUV::positions.size()). */
+fornos_status fornos_meshmap_texel_count(const fornos_meshmap *mm, uint32_t *out);
+void fornos_meshmap_destroy(fornos_meshmap *mm);
+
+/* ---- solvers (Src/solver_*.h) ------------------------------------------ */
+#define FORNOS_SOLVER_HEIGHT 0        /* HeightSolver,      Src/solver_height.h      */
+#define FORNOS_SOLVER_POSITION 1      /* PositionSolver,    Src/solver_position.h    */
+#define FORNOS_SOLVER_NORMALS 2       /* NormalsSolver,     Src/solver_normals.h     */
+#define FORNOS_SOLVER_AO 3            /* AmbientOcclusionSolver, Src/solver_ao.h     */
+#define FORNOS_SOLVER_BENT_NORMALS 4  /* BentNormalsSolver, Src/solver_bentnormals.h */
+#define FORNOS_SOLVER_THICKNESS 5     /* ThicknessSolver,   Src/solver_thickness.h   */
+
+/* Union of the reference's solver Params structs. */
+typedef struct fornos_solver_params {
+    uint32_t sample_count;   /* AO / BN / thickness: sampleCount */
+    float min_distance;      /* AO / BN / thickness: minDistance */
+    float max_distance;      /* AO / BN / thickness / height: maxDistance */
+    int tangent_space;       /* normals / BN: tangentSpace */
+    int normalize_output;    /* height: normalizeOutput (used at export only) */
+} fornos_solver_params;
+
+typedef struct fornos_solver fornos_solver;
+
+/* XSolver(params) + init(map, meshMapping) (e.g. Src/solver_ao.cpp:47-79).
+ * sample_table: n_perm * sample_count directions, 3 floats each, indexed
+ * (texel % n_perm) * sample_count + s (Shaders/ao_step1.comp:153); the
+ * reference draws it with computeSamplesImportanceCosDir (Src/math.h:363-382,
+ * n_perm = 4096); here the caller passes it as data.  Ignored (may be NULL)
+ * for height/position/normals.  sample_count must be in [1, 4096]. */
+fornos_status fornos_solver_create(int kind, const fornos_solver_params *params,
+                                   fornos_meshmap *mm, const float *sample_table,
+                                   uint32_t n_perm, fornos_solver **out);
+/* XSolver::runStep: one chunk of at most 131072 work items (rays for the
+ * hemisphere solvers, texels otherwise), Src/solver_ao.cpp:81-131. */
+fornos_status fornos_solver_run_step(fornos_solver *s, int *done);
+/* All remaining work in one launch; *ms (may be NULL) = kernel time. */
+fornos_status fornos_solver_run_all(fornos_solver *s, float *ms);
+fornos_status fornos_solver_progress(const fornos_solver *s, float *out);
+/* Rewinds progress to 0 so the next run_step/run_all repeats the whole pass
+ * (same inputs, same results). */
+fornos_status fornos_solver_reset(fornos_solver *s);
+/* XSolver::getResults (Src/solver_ao.cpp:133-138): copies count values into
+ * caller memory: 1 float per texel (height, AO, thickness) or 3 (position,
+ * normals, bent normals).  bytes must be at least that size. */
+fornos_status fornos_solver_read(const fornos_solver *s, void *out, size_t bytes);
+/* Per-texel ray counts of the last hemisphere run: occluded rays (AO, BN) or
+ * rays that hit (thickness).  count uint32 entries. */
+fornos_status fornos_solver_read_counts(const fornos_solver *s, uint32_t *out);
+/* Traversal work of the last run_all of a hemisphere solver, counted by an
+ * instrumented run of the same kernel: node tests, triangle tests, rays. */
+fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
+                                       uint64_t *tri_tests, uint64_t *rays);
+void fornos_solver_destroy(fornos_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

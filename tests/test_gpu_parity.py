@@ -1,0 +1,191 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Same seeded inputs on both sides; the oracle builds its own BVH and flatten
+(restated from Src/bvh.cpp and Src/meshmapping.cpp:62-125) and runs the
+restated shaders.  Integer and index results must be identical; float results
+are compared bit for bit, because both sides follow the same operation order
+with IEEE-correct division and square root and no FMA contraction.
+"""
+import numpy as np
+import pytest
+
+import fornos_amd as fa
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xFFFFFFFF
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def same_bits(a, b):
+    """Bitwise float equality, with every NaN equal to every NaN."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    nan = np.isnan(a) & np.isnan(b)
+    return np.array_equal(bits(a)[~nan], bits(b)[~nan]) and np.array_equal(np.isnan(a), np.isnan(b))
+
+
+class Bake:
+    """A prepared workload with its oracle-side BVH."""
+
+    def __init__(self, cfg, cull=True):
+        self.prep = fa.prepare(cfg)
+        self.cfg = self.prep["cfg"]
+        self.tex = self.prep["texels"].arrays()
+        self.scene = self.prep["scene"]
+        hi = self.prep["high"].arrays()
+        self.nodes, self.spos, self.snrm, self.stri = orc.bvh_build(hi)
+        self.mm = fa.MeshMapping().init(self.prep["texels"], self.scene, cull)
+        self.ms = self.mm.run_all()
+        self.coords, self.tidx = self.mm.read()
+
+    def oracle_map(self, begin=0, end=None, cull=True):
+        return orc.meshmapping(self.tex["positions"], self.tex["directions"], self.spos, self.nodes, cull,
+                               begin=begin, end=end)
+
+
+@pytest.fixture(scope="module")
+def sphere(gpu):
+    return Bake("sphere512")
+
+
+@pytest.fixture(scope="module")
+def field(gpu):
+    return Bake(fa.configs.scaled("heightfield2048", size=256, high_quads=160))
+
+
+def test_scene_matches_oracle_flatten(sphere):
+    nodes, ids = sphere.scene.reference_nodes()
+    assert np.array_equal(nodes, sphere.nodes)
+    assert np.array_equal(ids, sphere.stri)
+    sp, sn = sphere.scene.soup()
+    assert same_bits(sp, sphere.spos) and same_bits(sn, sphere.snrm)
+
+
+@pytest.mark.parametrize("which", ["sphere", "field"])
+def test_meshmapping_culling_matches_oracle(which, request):
+    b = request.getfixturevalue(which)
+    coords, tidx = b.oracle_map(cull=True)
+    n = len(tidx)
+    miss = int(np.sum(tidx == MISS))
+    print("%s: %d texels, %d misses, mapping %.3f ms" % (which, n, miss, b.ms))
+    assert np.array_equal(b.tidx, tidx)
+    assert same_bits(b.coords, coords)
+    assert miss < n // 10
+
+
+def test_meshmapping_nocull_matches_oracle(sphere):
+    mm = fa.MeshMapping().init(sphere.prep["texels"], sphere.scene, False)
+    mm.run_all()
+    c, t = mm.read()
+    oc, ot = sphere.oracle_map(cull=False)
+    assert np.array_equal(t, ot)
+    assert same_bits(c, oc)
+
+
+def test_meshmapping_steps_equal_run_all(field):
+    mm = fa.MeshMapping().init(field.prep["texels"], field.scene, True)
+    steps = 0
+    while not mm.runStep():
+        steps += 1
+    assert mm.progress() == 1.0
+    c, t = mm.read()
+    assert np.array_equal(t, field.tidx) and same_bits(c, field.coords)
+
+
+def test_mapping_hits_agree_with_brute_force(sphere):
+    # BVH-independent: the closest hit over all triangles, no tree.
+    sel = np.arange(0, len(sphere.tidx), 997)
+    bc, bt = orc.map_bruteforce(sphere.tex["positions"][sel], sphere.tex["directions"][sel], sphere.spos, True)
+    gt = sphere.tidx[sel]
+    ids_gpu = np.where(gt == MISS, -1, sphere.stri[np.minimum(gt, len(sphere.stri) * 3 - 1) // 3].astype(np.int64))
+    ids_bf = np.where(bt == MISS, -1, sphere.stri[np.minimum(bt, len(sphere.stri) * 3 - 1) // 3].astype(np.int64))
+    assert np.mean(ids_gpu == ids_bf) > 0.999
+    ok = (gt != MISS) & (bt != MISS)
+    diag = 2 * np.sqrt(3) * 1.2
+    assert np.max(np.abs(sphere.coords[sel][ok, 0] - bc[ok, 0])) <= 1e-5 * diag
+
+
+def test_height_position_normals_match_oracle(sphere, field):
+    for b, tangent in ((sphere, False), (field, True)):
+        h = fa.HeightSolver(True, 0.0).init(b.prep["texels"], b.mm)
+        h.run_all()
+        assert same_bits(h.getResults(), orc.height(b.coords))
+        p = fa.PositionSolver().init(b.prep["texels"], b.mm)
+        p.run_all()
+        assert same_bits(p.getResults(), orc.texel_attr(0, b.coords, b.tidx, b.spos))
+        nsv = fa.NormalsSolver(tangent).init(b.prep["texels"], b.mm)
+        nsv.run_all()
+        expect = orc.texel_attr(1, b.coords, b.tidx, b.snrm)
+        if tangent:
+            hit = b.tidx != MISS
+            expect[hit] = orc.tangentspace(b.tex["normals"], b.tex["tangents"], b.tex["bitangents"], expect)[hit]
+        assert same_bits(nsv.getResults(), expect)
+
+
+def _ranges(n, k=3, width=1024):
+    starts = np.linspace(0, max(0, n - width), k).astype(int)
+    return [(int(s), int(min(n, s + width))) for s in starts]
+
+
+def _hemi_check(b, solver, kind, S, mind, maxd, tangent=False, width=1024):
+    res = solver.getResults()
+    cnt = solver.counts()
+    table = solver.table
+    for lo, hi in _ranges(len(b.tidx), width=width):
+        out, oc = orc.hemisphere(kind, S, mind, maxd, table, b.coords, b.tidx, b.spos, b.snrm, b.nodes, lo, hi)
+        if kind == orc.KIND_BN and tangent:
+            hit = b.tidx[lo:hi] != MISS
+            out[hit] = orc.tangentspace(b.tex["normals"], b.tex["tangents"], b.tex["bitangents"], out, begin=lo)[hit]
+        assert np.array_equal(cnt[lo:hi], oc), "per-texel ray counts differ in [%d,%d)" % (lo, hi)
+        assert same_bits(res[lo:hi], out), "values differ in [%d,%d)" % (lo, hi)
+
+
+def test_ao_matches_oracle_sphere(sphere):
+    p = sphere.cfg["maps"]["ao"]
+    s = fa.AmbientOcclusionSolver(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(
+        sphere.prep["texels"], sphere.mm)
+    ms = s.run_all()
+    print("sphere AO-32: %.3f ms" % ms)
+    _hemi_check(sphere, s, orc.KIND_AO, p["sampleCount"], p["minDistance"], p["maxDistance"], width=2048)
+    ao = s.getResults()
+    assert np.all((ao >= 0) & (ao <= 1))
+
+
+@pytest.mark.parametrize("kind", ["ao", "bentnormals", "thickness"])
+def test_hemisphere_matches_oracle_field(field, kind):
+    p = field.cfg["maps"][kind]
+    cls = {"ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
+           "thickness": fa.ThicknessSolver}[kind]
+    kw = {"tangentSpace": p["tangentSpace"]} if kind == "bentnormals" else {}
+    s = cls(p["sampleCount"], p["minDistance"], p["maxDistance"], **kw).init(field.prep["texels"], field.mm)
+    s.run_all()
+    k = {"ao": orc.KIND_AO, "bentnormals": orc.KIND_BN, "thickness": orc.KIND_THICK}[kind]
+    _hemi_check(field, s, k, p["sampleCount"], p["minDistance"], p["maxDistance"],
+                tangent=p.get("tangentSpace", False), width=256)
+
+
+def test_hemisphere_steps_equal_run_all(field):
+    p = field.cfg["maps"]["thickness"]
+    a = fa.ThicknessSolver(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
+    a.run_all()
+    b = fa.ThicknessSolver(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
+    n = 0
+    while not b.runStep():
+        n += 1
+    assert n >= 1
+    assert same_bits(a.getResults(), b.getResults())
+
+
+@pytest.mark.parametrize("S", [1, 7, 64, 300, 1024])
+def test_ao_sample_counts(field, S):
+    s = fa.AmbientOcclusionSolver(S, 0.01, 0.2).init(field.prep["texels"], field.mm)
+    s.run_all()
+    res, cnt = s.getResults(), s.counts()
+    out, oc = orc.hemisphere(orc.KIND_AO, S, 0.01, 0.2, s.table, field.coords, field.tidx, field.spos,
+                             field.snrm, field.nodes, 0, 128)
+    assert np.array_equal(cnt[:128], oc) and same_bits(res[:128], out)

@@ -1,0 +1,112 @@
+"""Host-side input producers of the product against the oracle (CPU only).
+
+Normals, tangent frames, the MapUV raster + CompressedMapUV, the sample table
+and the reference-layout BVH + flatten must be bit-identical to the oracle's
+restatement of the reference functions.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fornos_amd as fa
+import oracle as orc
+from fornos_amd._abi import check, lib, ptr
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def eq_nan(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    n = np.isnan(a) & np.isnan(b)
+    return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(bits(a)[~n], bits(b)[~n])
+
+
+@pytest.fixture(scope="module", params=["sphere512", "heightfield_small", "knot_small"])
+def prep(request):
+    if request.param == "sphere512":
+        cfg = fa.CONFIGS["sphere512"]
+    elif request.param == "heightfield_small":
+        cfg = fa.configs.scaled("heightfield2048", size=384, high_quads=96)
+    else:
+        cfg = fa.configs.scaled("knot4096", size=256, high_quads=64)
+    return fa.prepare(cfg, with_scene=False)
+
+
+def test_vertex_normals(prep):
+    h = prep["high"].arrays()
+    n, vn = orc.vertex_normals(h["positions"], h["vertices"], h["triangles"])
+    assert np.array_equal(bits(n), bits(h["normals"]))
+    assert np.array_equal(vn, h["vertices"][:, 2])
+
+
+def test_aggressive_normals(prep):
+    lo, d = prep["low"].arrays(), prep["dirs"].arrays()
+    n, vn = orc.vertex_normals_aggressive(lo["positions"], lo["vertices"], lo["triangles"])
+    assert np.array_equal(bits(n), bits(d["normals"]))
+    assert np.array_equal(vn, d["vertices"][:, 2])
+
+
+def test_tangent_space(prep):
+    lo = prep["low"].arrays()
+    if "tangents" not in lo:
+        pytest.skip("workload bakes no tangent-space map")
+    t, b = orc.tangent_space(lo["positions"], lo["normals"], lo["texcoords"], lo["vertices"], lo["triangles"])
+    assert eq_nan(t, lo["tangents"]) and eq_nan(b, lo["bitangents"])
+
+
+def test_mapuv_and_compress(prep):
+    lo, d, tex = prep["low"].arrays(), prep["dirs"].arrays(), prep["texels"]
+    W = tex.width
+    m = orc.mapuv(lo, W, W, d["normals"], d["vertices"][:, 2], lo.get("tangents"), lo.get("bitangents"))
+    pm = tex.mapuv()
+    for k in m:
+        assert eq_nan(m[k], pm[k]), k
+    idx = orc.compress(m["directions"])
+    ta = tex.arrays()
+    assert np.array_equal(idx, ta["indices"])
+    assert eq_nan(m["positions"][idx], ta["positions"])
+    assert eq_nan(m["directions"][idx], ta["directions"])
+
+
+def test_reference_bvh_matches_oracle(prep):
+    hi = prep["high"]
+    n = C.c_uint32()
+    check(lib().fornos_bvh_reference_export(hi.handle, 8, 8192, C.byref(n), None, None))
+    nodes = np.empty((n.value, 9), np.uint32)
+    ids = np.empty(hi.n_triangles, np.uint32)
+    check(lib().fornos_bvh_reference_export(hi.handle, 8, 8192, C.byref(n), ptr(nodes), ptr(ids)))
+    on, sp, sn, st = orc.bvh_build(hi.arrays())
+    assert np.array_equal(on, nodes)
+    assert np.array_equal(st, ids)
+
+
+@pytest.mark.parametrize("S", [1, 32, 256, 1000])
+def test_sample_table(S):
+    tab, rnd = fa.sample_table(S, 64, seed=99)
+    assert np.array_equal(bits(orc.samples(S, rnd)), bits(tab))
+
+
+def test_sample_table_seeded_and_deterministic():
+    a, ra = fa.sample_table(32, 16, seed=5)
+    b, rb = fa.sample_table(32, 16, seed=5)
+    c, rc = fa.sample_table(32, 16, seed=6)
+    assert np.array_equal(a, b) and np.array_equal(ra, rb)
+    assert not np.array_equal(ra, rc)
+
+
+def test_generator_sizes():
+    assert fa.Mesh.generate(0, 0, 1.0, 32, 16).n_triangles == 960
+    assert fa.Mesh.generate(1, 7, 0.02, 6, 0).n_triangles == 81920
+    assert fa.Mesh.generate(3, 1, 0.05, 64, 0).n_triangles == 8192
+    assert fa.Mesh.generate(2, 1, 0.05, 32, 10).n_triangles == 2 * 32 * 32
+
+
+def test_generators_deterministic():
+    a = fa.Mesh.generate(2, 3, 0.05, 48, 20).arrays()
+    b = fa.Mesh.generate(2, 3, 0.05, 48, 20).arrays()
+    c = fa.Mesh.generate(2, 4, 0.05, 48, 20).arrays()
+    assert np.array_equal(a["positions"], b["positions"])
+    assert not np.array_equal(a["positions"], c["positions"])
