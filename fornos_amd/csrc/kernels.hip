@@ -156,10 +156,52 @@ __global__ __launch_bounds__(kBlock) void k_meshmap(const float4* __restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Hemisphere passes: ao_step0 (ray basis) + ao/bentnormals/thick step1 (one
-// ray per sample) + step2 (per-texel reduction) + tangentspace, fused.
+// Hemisphere passes (AO, bent normals, thickness).
+//
+// k_rays: persistent waves.  Each lane owns one ray (ao_step0 ray basis +
+// ao/bentnormals/thick step1 traversal); a lane whose ray has finished takes
+// a new one from a global counter, one atomic per wave for all its idle
+// lanes, so a wave never idles behind its slowest ray.  Every ray writes one
+// result (a hit byte, or t for thickness) to a per-launch buffer; k_reduce
+// then folds each texel's S results in sample order (step2 + tangentspace),
+// the same split as the reference's step1/step2 dispatches.
 // ---------------------------------------------------------------------------
 enum { kAO = 0, kBN = 1, kThick = 2 };
+
+constexpr uint32_t kRefill = 16;           // refill a wave once this many lanes are idle
+constexpr float kSlabRel = 1.0f / 2097152.0f;  // 2^-21 = 8 units in the last place
+constexpr float kSlabAbs = 1e-37f;
+constexpr float kSlowDir = 9.094947e-13f;  // 2^-40: smaller direction components use the exact test
+
+struct Ray {
+    V3 o, d, inv;
+    bool slow;
+};
+
+// The reference's node test is RayAABB(o, d, box) < T with six IEEE
+// divisions (Shaders/ao_step1.comp:43-52, :176).  Here the slab distances are
+// first taken as (m - o) * RN(1/d), which lies within 3 units in the last
+// place of RN((m - o)/d) (two roundings of a correctly rounded reciprocal and
+// product against one of the quotient); max/min keep that relative bound.
+// With an 8-ulp margin the outcome is decided whenever it cannot flip, and
+// only the remaining boxes are retested with the reference's divisions, so
+// the decision is always the reference's.
+__device__ inline bool node_visit(const Ray& r, float4 lo, float4 hi, float T)
+{
+    if (!r.slow) {
+        const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
+        const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
+        const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
+        const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+        const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+        const float ea = __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
+        const float eb = __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
+        const float al = a - ea, ah = a + ea, bl = b - eb, bh = b + eb;
+        if (bl >= 0.0f && ah <= bl && ah < T) return true;
+        if (bh < 0.0f || al > bh || al >= T) return false;
+    }
+    return ray_aabb(r.o, r.d, lo, hi) < T;
+}
 
 // Shaders/ao_step1.comp:109-147 with the per-triangle terms read from `tri`.
 // Returns t, or FLT_MAX.  Triangles whose t falls outside [tmin, tmax) cannot
@@ -185,182 +227,215 @@ __device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, flo
     return (x >= 0 && y >= 0 && y <= 1 && z >= 0 && z <= 1) ? t : FLT_MAX;
 }
 
-// raycastBVH of Shaders/ao_step1.comp:166-193 in the reference order.
-// AO  : any hit with t in [minD, maxD) decides the ray (ao_step1.comp:158);
-//       nodes are visited iff entry < maxD (the running minimum can only
-//       prune nodes that the maxD test already prunes).
-// BN  : any hit with t >= minD decides the ray (bentnormals_step1.comp:170).
-// Thick: closest hit with t >= minD among nodes with entry < min(best, maxD).
-template <int KIND, bool COUNT>
-__device__ inline float trace(V3 o, V3 d, float mind, float maxd, const DevScene& sc, uint32_t& cn,
-                              uint32_t& ct)
+struct Basis {
+    V3 o, d, tx, ty;
+};
+
+// Shaders/ao_step0.comp:39-56 for a texel whose mapping hit soup vertex t.
+__device__ inline Basis ray_basis(const DevScene& sc, float4 c, uint32_t t)
 {
-    float best = FLT_MAX;
-    const float tri_hi = KIND == kAO ? maxd : (KIND == kBN ? INFINITY : FLT_MAX);
-    uint32_t i = 0;
-    while (i < sc.n_nodes) {
-        const float4 lo = sc.nodes[2 * i], hi = sc.nodes[2 * i + 1];
-        const uint32_t w = __float_as_uint(hi.w);
-        if (COUNT) ++cn;
-        const float da = ray_aabb(o, d, lo, hi);
-        if (da < best && da < maxd) {
-            if (w & kLeafFlag) {
-                const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
-                for (uint32_t k = first; k < end; ++k) {
-                    if (COUNT) ++ct;
-                    const float t = hemi_tri(o, d, k, sc, mind, KIND == kThick ? best : tri_hi);
-                    if (t != FLT_MAX) {
-                        if (KIND != kThick) return t;
-                        best = t;
-                    }
-                }
-            }
-            ++i;
-        } else {
-            i = (w & kLeafFlag) ? i + 1 : w;
-        }
-    }
-    return best;
+    Basis B;
+    const V3 p0 = xyz(sc.spos[t]), p1 = xyz(sc.spos[t + 1]), p2 = xyz(sc.spos[t + 2]);
+    B.o = p0 * c.y + p1 * c.z + p2 * c.w;
+    const V3 n0 = xyz(sc.snrm[t]), n1 = xyz(sc.snrm[t + 1]), n2 = xyz(sc.snrm[t + 2]);
+    B.d = normalize(n0 * c.y + n1 * c.z + n2 * c.w);
+    const V3 d = B.d;
+    B.ty = normalize(fabsf(d.x) > fabsf(d.y) ? mk(d.z, 0, -d.x) : mk(0, d.z, -d.y));
+    B.tx = cross(d, B.ty);
+    return B;
 }
 
-struct HemiArgs {
+// Sample direction of ao_step1.comp:153-155 (thick_step1.comp:144-150 flips d).
+__device__ inline V3 sample_dir(const Basis& B, V3 d, float4 rs)
+{
+    return normalize(B.tx * rs.x + B.ty * rs.y + d * rs.z);
+}
+
+struct RayArgs {
     DevScene sc;
     const float4* __restrict__ coords;
     const uint32_t* __restrict__ tidx;
     const float4* __restrict__ samples;
-    const float4* __restrict__ pixt;   // n, t, b per texel (BN tangent space) or null
-    float* __restrict__ out;           // 1 or 3 floats per texel (global texel index)
-    uint32_t* __restrict__ counts;     // per texel
-    unsigned long long* __restrict__ stats;  // node tests, tri tests, rays (COUNT only)
-    uint32_t S, P, t0, t1, tpb;
+    uint8_t* __restrict__ hit;          // AO / BN: 1 when the ray hit (per ray of the launch)
+    float* __restrict__ tval;           // thickness: closest t, FLT_MAX when nothing was hit
+    unsigned int* __restrict__ counter;  // next ray of the launch to hand out
+    unsigned long long* __restrict__ stats;  // node tests, triangle tests, rays (COUNT only)
+    uint32_t S, P, t0, ntex;            // texels [t0, t0 + ntex), S rays each
     float mind, maxd;
-    int tangent;
 };
 
+// One step of raycastBVH (Shaders/ao_step1.comp:166-193) in the reference's
+// stackless order.  AO: any hit with t in [minD, maxD) decides the ray
+// (ao_step1.comp:158); nodes are visited iff entry < maxD (the running minimum
+// only prunes nodes the maxD test already prunes).  BN: any hit with
+// t >= minD decides it (bentnormals_step1.comp:170).  Thickness: closest hit
+// with t >= minD over nodes with entry < min(best, maxD).
 template <int KIND, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_hemi(HemiArgs A)
+__global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
 {
-    extern __shared__ float smem[];
-    // layout: basis[tpb][12], miss[tpb] (as uint), cnt[tpb] (uint), vals[...]
-    float* basis = smem;
-    uint32_t* miss = (uint32_t*)(basis + 12 * A.tpb);
-    uint32_t* cnt = miss + A.tpb;
-    float* vals = (float*)(cnt + A.tpb);
-    const uint32_t S = A.S, tpb = A.tpb, tid = threadIdx.x;
-    const uint32_t tex0 = A.t0 + blockIdx.x * tpb;
-
-    // ao_step0.comp:39-56
-    if (tid < tpb) {
-        const uint32_t g = tex0 + tid;
-        uint32_t m = 1;
-        if (g < A.t1) {
-            const uint32_t t = A.tidx[g];
-            if (t != kMiss) {
-                m = 0;
-                const float4 c = A.coords[g];
-                const DevScene& sc = A.sc;
-                const V3 p0 = xyz(sc.spos[t]), p1 = xyz(sc.spos[t + 1]), p2 = xyz(sc.spos[t + 2]);
-                const V3 o = p0 * c.y + p1 * c.z + p2 * c.w;
-                const V3 n0 = xyz(sc.snrm[t]), n1 = xyz(sc.snrm[t + 1]), n2 = xyz(sc.snrm[t + 2]);
-                const V3 d = normalize(n0 * c.y + n1 * c.z + n2 * c.w);
-                const V3 ty = normalize(fabsf(d.x) > fabsf(d.y) ? mk(d.z, 0, -d.x) : mk(0, d.z, -d.y));
-                const V3 tx = cross(d, ty);
-                const V3 dd = KIND == kThick ? -d : d;   // thick_step1.comp:144
-                float* b = &basis[12 * tid];
-                b[0] = o.x; b[1] = o.y; b[2] = o.z; b[3] = dd.x; b[4] = dd.y; b[5] = dd.z;
-                b[6] = tx.x; b[7] = tx.y; b[8] = tx.z; b[9] = ty.x; b[10] = ty.y; b[11] = ty.z;
+    const uint32_t total = A.ntex * A.S;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t n_nodes = A.sc.n_nodes;
+    Ray ray;
+    ray.o = ray.d = ray.inv = mk(0, 0, 0);
+    ray.slow = false;
+    uint32_t rid = 0, node = 0;
+    float best = FLT_MAX;
+    bool active = false, exhausted = false;
+    uint32_t cn = 0, ct = 0, nr = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = __popcll(idle);
+        if (!exhausted && (nidle >= kRefill || nidle == 64)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(A.counter, nidle);
+            base = __shfl(base, 0);
+            if (base + nidle >= total) exhausted = true;
+            if (!active) {
+                const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+                if (r < total) {
+                    const uint32_t q = r / A.S, s = r - q * A.S, g = A.t0 + q;
+                    const uint32_t t = A.tidx[g];
+                    if (t == kMiss) {
+                        if (KIND == kThick) A.tval[r] = FLT_MAX;
+                        else A.hit[r] = 0;
+                    } else {
+                        const Basis B = ray_basis(A.sc, A.coords[g], t);
+                        const V3 d = KIND == kThick ? -B.d : B.d;
+                        ray.o = B.o;
+                        ray.d = sample_dir(B, d, A.samples[(size_t)(g % A.P) * A.S + s]);
+                        ray.inv = mk(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
+                        ray.slow = !(fabsf(ray.d.x) >= kSlowDir && fabsf(ray.d.y) >= kSlowDir &&
+                                     fabsf(ray.d.z) >= kSlowDir);
+                        rid = r;
+                        node = 0;
+                        best = FLT_MAX;
+                        active = true;
+                        if (COUNT) ++nr;
+                    }
+                }
             }
         }
-        miss[tid] = m;
-        cnt[tid] = 0;
-    }
-    __syncthreads();
-
-    uint32_t cn = 0, ct = 0, nrays = 0;
-    const uint32_t passes = S <= kBlock ? 1 : (S + kBlock - 1) / kBlock;
-    for (uint32_t pass = 0; pass < passes; ++pass) {
-        uint32_t lt, s;
-        if (S <= kBlock) { lt = tid / S; s = tid - lt * S; }
-        else { lt = 0; s = pass * kBlock + tid; }
-        if (lt < tpb && s < S && !miss[lt] && tex0 + lt < A.t1) {
-            const uint32_t g = tex0 + lt;
-            const float* b = &basis[12 * lt];
-            const V3 o = mk(b[0], b[1], b[2]), d = mk(b[3], b[4], b[5]);
-            const V3 tx = mk(b[6], b[7], b[8]), ty = mk(b[9], b[10], b[11]);
-            const float4 rs = A.samples[(size_t)(g % A.P) * S + s];   // ao_step1.comp:153
-            const V3 dir = normalize(tx * rs.x + ty * rs.y + d * rs.z);
-            const float t = trace<KIND, COUNT>(o, dir, A.mind, A.maxd, A.sc, cn, ct);
-            if (COUNT) ++nrays;
-            const uint32_t v = lt * S + s;
-            if (KIND == kAO) {
-                if (t != FLT_MAX && t < A.maxd) atomicAdd(&cnt[lt], 1u);
-            } else if (KIND == kBN) {
-                const bool hit = t != FLT_MAX;
-                if (hit) atomicAdd(&cnt[lt], 1u);
-                vals[3 * v] = hit ? 0.0f : dir.x;
-                vals[3 * v + 1] = hit ? 0.0f : dir.y;
-                vals[3 * v + 2] = hit ? 0.0f : dir.z;
+        if (exhausted && __ballot(active) == 0) break;
+        if (active) {
+            const float4 lo = A.sc.nodes[2 * node], hi = A.sc.nodes[2 * node + 1];
+            const uint32_t w = __float_as_uint(hi.w);
+            const float T = KIND == kThick ? fminf(best, A.maxd) : A.maxd;
+            bool done = false, found = false;
+            if (COUNT) ++cn;
+            if (node_visit(ray, lo, hi, T)) {
+                if (w & kLeafFlag) {
+                    const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
+                    const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : best);
+                    for (uint32_t k = first; k < end; ++k) {
+                        if (COUNT) ++ct;
+                        const float t = hemi_tri(ray.o, ray.d, k, A.sc, A.mind, KIND == kThick ? best : tri_hi);
+                        if (t != FLT_MAX) {
+                            if (KIND != kThick) { found = true; break; }
+                            best = t;
+                        }
+                    }
+                }
+                ++node;
             } else {
-                const bool hit = t != FLT_MAX;
-                if (hit) atomicAdd(&cnt[lt], 1u);
-                vals[v] = hit ? t : A.maxd;
+                node = (w & kLeafFlag) ? node + 1 : w;
+            }
+            if (found || node >= n_nodes) done = true;
+            if (done) {
+                if (KIND == kThick) A.tval[rid] = best;
+                else A.hit[rid] = found ? 1 : 0;
+                active = false;
             }
         }
     }
     if (COUNT) {
-        // one 64-bit atomic per wave per counter
-        unsigned long long a = cn, b = ct, c = nrays;
+        unsigned long long a = cn, b = ct, c = nr;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_down(a, off);
             b += __shfl_down(b, off);
             c += __shfl_down(c, off);
         }
-        if ((tid & 63) == 0) {
+        if (lane == 0) {
             atomicAdd(&A.stats[0], a);
             atomicAdd(&A.stats[1], b);
             atomicAdd(&A.stats[2], c);
         }
     }
-    __syncthreads();
-
-    // step2 reductions in sample order (ao_step2.comp:24-29, thick_step2.comp:24-29,
-    // bentnormals_step2.comp:25-34) and tangentspace.comp:22-41.
-    if (tid < tpb) {
-        const uint32_t g = tex0 + tid;
-        if (g < A.t1) {
-            if (A.counts) A.counts[g] = cnt[tid];
-            if (KIND == kAO) {
-                A.out[g] = miss[tid] ? 1.0f : 1.0f - (float)cnt[tid] / (float)S;
-            } else if (KIND == kThick) {
-                float acc = 0;
-                if (!miss[tid])
-                    for (uint32_t s = 0; s < S; ++s) acc += vals[tid * S + s];
-                A.out[g] = miss[tid] ? A.maxd : acc / (float)S;
-            } else {
-                V3 r = mk(0, 0, 0);
-                if (!miss[tid]) {
-                    V3 acc = mk(0, 0, 0);
-                    for (uint32_t s = 0; s < S; ++s) {
-                        const float* v = &vals[3 * (tid * S + s)];
-                        acc = acc + mk(v[0], v[1], v[2]);
-                    }
-                    r = normalize(acc);
-                    if (A.tangent) {
-                        const V3 n = xyz(A.pixt[3 * g]), t = xyz(A.pixt[3 * g + 1]), b = xyz(A.pixt[3 * g + 2]);
-                        const V3 d0 = mk(n.z * b.y - n.y * b.z, n.x * b.z - n.z * b.x, n.y * b.x - n.x * b.y);
-                        const V3 d1 = mk(t.z * n.y - t.y * n.z, t.x * n.z - n.x * t.z, n.x * t.y - t.x * n.y);
-                        const V3 d2 = mk(t.y * b.z - t.z * b.y, t.z * b.x - t.x * b.z, t.x * b.y - t.y * b.x);
-                        r = normalize(mk(dot(r, d0), dot(r, d1), dot(r, d2)));
-                    }
-                }
-                A.out[3 * g] = r.x;
-                A.out[3 * g + 1] = r.y;
-                A.out[3 * g + 2] = r.z;
-            }
-        }
-    }
 }
 
+struct ReduceArgs {
+    DevScene sc;
+    const float4* __restrict__ coords;
+    const uint32_t* __restrict__ tidx;
+    const float4* __restrict__ samples;
+    const float4* __restrict__ pixt;    // n, t, b per texel (BN tangent space) or null
+    const uint8_t* __restrict__ hit;
+    const float* __restrict__ tval;
+    float* __restrict__ out;            // 1 or 3 floats per texel (global texel index)
+    uint32_t* __restrict__ counts;      // per texel
+    uint32_t S, P, t0, ntex;
+    float maxd;
+    int tangent;
+};
+
+// step2 reductions in sample order (ao_step2.comp:24-29, thick_step2.comp:24-29,
+// bentnormals_step2.comp:25-34) and tangentspace.comp:22-41.  Texels whose
+// mapping missed get AO 1, thickness maxD and bent normal (0,0,0).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= A.ntex) return;
+    const uint32_t g = A.t0 + q, S = A.S;
+    const size_t base = (size_t)q * S;
+    const uint32_t t = A.tidx[g];
+    uint32_t cnt = 0;
+    if (KIND == kAO) {
+        if (t != kMiss)
+            for (uint32_t s = 0; s < S; ++s) cnt += A.hit[base + s];
+        A.out[g] = t == kMiss ? 1.0f : 1.0f - (float)cnt / (float)S;
+    } else if (KIND == kThick) {
+        float acc = 0;
+        if (t != kMiss)
+            for (uint32_t s = 0; s < S; ++s) {
+                const float v = A.tval[base + s];
+                if (v != FLT_MAX) ++cnt;
+                acc += v != FLT_MAX ? v : A.maxd;
+            }
+        A.out[g] = t == kMiss ? A.maxd : acc / (float)S;
+    } else {
+        V3 r = mk(0, 0, 0);
+        if (t != kMiss) {
+            const Basis B = ray_basis(A.sc, A.coords[g], t);
+            const float4* smp = A.samples + (size_t)(g % A.P) * S;
+            V3 acc = mk(0, 0, 0);
+            for (uint32_t s = 0; s < S; ++s) {
+                if (A.hit[base + s]) { ++cnt; continue; }   // adds vec3(0): acc never holds -0
+                acc = acc + sample_dir(B, B.d, smp[s]);
+            }
+            r = normalize(acc);
+            if (A.tangent) {
+                const V3 n = xyz(A.pixt[3 * g]), tt = xyz(A.pixt[3 * g + 1]), b = xyz(A.pixt[3 * g + 2]);
+                const V3 d0 = mk(n.z * b.y - n.y * b.z, n.x * b.z - n.z * b.x, n.y * b.x - n.x * b.y);
+                const V3 d1 = mk(tt.z * n.y - tt.y * n.z, tt.x * n.z - n.x * tt.z, n.x * tt.y - tt.x * n.y);
+                const V3 d2 = mk(tt.y * b.z - tt.z * b.y, tt.z * b.x - tt.x * b.z, tt.x * b.y - tt.y * b.x);
+                r = normalize(mk(dot(r, d0), dot(r, d1), dot(r, d2)));
+            }
+        }
+        A.out[3 * g] = r.x;
+        A.out[3 * g + 1] = r.y;
+        A.out[3 * g + 2] = r.z;
+    }
+    A.counts[g] = cnt;
+}
+
+// Clears n 32-bit words (a kernel rather than hipMemsetAsync, so the whole
+// pass can be captured into a HIP graph).
+__global__ void k_zero(uint32_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
 // ---------------------------------------------------------------------------
 // Texel passes: heights.comp, positions.comp, normals.comp (+ tangentspace).
 // ---------------------------------------------------------------------------
@@ -479,6 +554,7 @@ uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 struct fornos_scene {
     int device = 0;
+    uint32_t persistent_blocks = 2048;  // k_rays grid: 8 blocks of 256 lanes per CU
     hipStream_t stream = nullptr;
     DevScene d{};
     float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr;
@@ -517,10 +593,14 @@ struct fornos_solver {
     float* out = nullptr;
     uint32_t* counts = nullptr;
     unsigned long long* stats = nullptr;
+    void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
+    uint32_t chunk_texels = 0;         // texels per k_rays launch
+    uint32_t* counter = nullptr;       // k_rays work counter
     ~fornos_solver()
     {
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
         dev_free(samples); dev_free(out); dev_free(counts); dev_free(stats);
+        dev_free(raybuf); dev_free(counter);
     }
     bool hemi() const { return kind >= FORNOS_SOLVER_AO; }
     uint32_t comps() const
@@ -537,35 +617,48 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
 {
     fornos_meshmap* mm = s->mm;
     fornos_scene* sc = mm->sc;
-    HemiArgs A;
+    const uint32_t S = s->p.sample_count;
+    RayArgs A;
     A.sc = sc->d;
     A.coords = mm->coords;
     A.tidx = mm->tidx;
     A.samples = s->samples;
-    A.pixt = mm->pixt;
-    A.out = s->out;
-    A.counts = s->counts;
+    A.hit = KIND == kThick ? nullptr : (uint8_t*)s->raybuf;
+    A.tval = KIND == kThick ? (float*)s->raybuf : nullptr;
+    A.counter = s->counter;
     A.stats = s->stats;
-    A.S = s->p.sample_count;
+    A.S = S;
     A.P = s->P;
-    A.t0 = t0;
-    A.t1 = t1;
-    A.tpb = A.S <= (uint32_t)kBlock ? kBlock / A.S : 1;
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
-    A.tangent = s->p.tangent_space;
-    const uint32_t comps = KIND == kBN ? 3 : (KIND == kThick ? 1 : 0);
-    const size_t vals = (size_t)std::max<uint32_t>(kBlock, A.S) * comps;
-    const size_t shmem = sizeof(float) * (14 * (size_t)A.tpb + vals);
-    const uint32_t blocks = div_up(t1 - t0, A.tpb);
-    if (blocks == 0) return FORNOS_OK;
-    if (count) {
-        HIP_TRY(hipMemsetAsync(s->stats, 0, 3 * sizeof(unsigned long long), sc->stream));
-        hipLaunchKernelGGL((k_hemi<KIND, true>), dim3(blocks), dim3(kBlock), shmem, sc->stream, A);
-    } else {
-        hipLaunchKernelGGL((k_hemi<KIND, false>), dim3(blocks), dim3(kBlock), shmem, sc->stream, A);
+    ReduceArgs R;
+    R.sc = sc->d;
+    R.coords = mm->coords;
+    R.tidx = mm->tidx;
+    R.samples = s->samples;
+    R.pixt = mm->pixt;
+    R.hit = A.hit;
+    R.tval = A.tval;
+    R.out = s->out;
+    R.counts = s->counts;
+    R.S = S;
+    R.P = s->P;
+    R.maxd = s->p.max_distance;
+    R.tangent = s->p.tangent_space;
+    if (count) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc->stream, (uint32_t*)s->stats, 6u);
+    for (uint32_t c0 = t0; c0 < t1; c0 += s->chunk_texels) {
+        const uint32_t nt = std::min(s->chunk_texels, t1 - c0);
+        A.t0 = R.t0 = c0;
+        A.ntex = R.ntex = nt;
+        const uint32_t blocks = std::min<uint32_t>(sc->persistent_blocks, div_up((uint64_t)nt * S, kBlock));
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc->stream, s->counter, 1u);
+        if (count)
+            hipLaunchKernelGGL((k_rays<KIND, true>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
+        else
+            hipLaunchKernelGGL((k_rays<KIND, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
+        hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
+        HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipGetLastError());
     return FORNOS_OK;
 }
 
@@ -634,6 +727,9 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
 
     std::unique_ptr<fornos_scene> sc(new fornos_scene());
     HIP_TRY(hipGetDevice(&sc->device));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sc->device));
+    sc->persistent_blocks = (uint32_t)std::max(1, cus) * 8;
     const auto t_begin = std::chrono::steady_clock::now();
     build_reference_bvh(*mesh, bp.max_tris_per_leaf, bp.max_depth, sc->bvh);
     const FlatBVH& B = sc->bvh;
@@ -839,6 +935,15 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         HIP_TRY(hipMemcpy(s->samples, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));
         if (fornos_status st = dev_alloc(&s->counts, mm->n)) return st;
         if (fornos_status st = dev_alloc(&s->stats, 3)) return st;
+        if (fornos_status st = dev_alloc(&s->counter, 1)) return st;
+        // per-ray results of one k_rays launch: at most 2^28 rays (256 MiB of
+        // hit bytes, 1 GiB of thickness t values)
+        s->chunk_texels = std::max<uint32_t>(1, (1u << 28) / S);
+        const size_t rays = (size_t)std::min<uint32_t>(s->chunk_texels, std::max<uint32_t>(1, mm->n)) * S;
+        const size_t elem = kind == FORNOS_SOLVER_THICKNESS ? sizeof(float) : 1;
+        uint8_t* rb = nullptr;
+        if (fornos_status st = dev_alloc(&rb, rays * elem)) return st;
+        s->raybuf = rb;
     } else {
         s->work_total = mm->work_count;
     }
