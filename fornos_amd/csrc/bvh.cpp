@@ -259,6 +259,179 @@ void export_reference_nodes(const FlatBVH& B, uint32_t* nodes9)
     }
 }
 
+std::vector<uint32_t> reference_parents(const FlatBVH& B)
+{
+    std::vector<uint32_t> parent(B.n_nodes, 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < B.n_nodes; ++i) {
+        const uint32_t w = B.nodes[8 * (size_t)i + 7];
+        if (w & kLeafFlag) continue;
+        for (uint32_t c = i + 1; c < w;) {
+            parent[c] = i;
+            const uint32_t cw = B.nodes[8 * (size_t)c + 7];
+            c = (cw & kLeafFlag) ? c + 1 : cw;
+        }
+    }
+    return parent;
+}
+
+namespace {
+
+float fbits(uint32_t w)
+{
+    float f;
+    std::memcpy(&f, &w, 4);
+    return f;
+}
+
+struct CullPrim {
+    V3 lo, hi, c;
+    uint32_t ref;  // reference leaf node
+};
+
+float true_area(V3 lo, V3 hi)
+{
+    const V3 s = hi - lo;
+    return 2.0f * (s.x * s.y + s.y * s.z + s.z * s.x);
+}
+
+struct CullNode {
+    V3 lo, hi;
+    int child = -1, right = -1;  // children in `pool`, or -1 for a leaf
+    uint32_t prim = 0;         // leaf: primitive index
+};
+
+struct CullBuilder {
+    std::vector<CullPrim>& prims;
+    std::vector<CullNode> pool;
+
+    // Binned SAH with the true surface area, 16 bins per axis, one primitive per leaf.
+    int build(uint32_t b, uint32_t e, uint32_t depth, uint32_t& max_depth)
+    {
+        max_depth = std::max(max_depth, depth);
+        const int id = (int)pool.size();
+        pool.emplace_back();
+        V3 lo = prims[b].lo, hi = prims[b].hi, clo = prims[b].c, chi = prims[b].c;
+        for (uint32_t i = b + 1; i < e; ++i) {
+            lo = hmin(lo, prims[i].lo); hi = hmax(hi, prims[i].hi);
+            clo = hmin(clo, prims[i].c); chi = hmax(chi, prims[i].c);
+        }
+        pool[id].lo = lo;
+        pool[id].hi = hi;
+        if (e - b == 1) { pool[id].prim = b; return id; }
+        const float ext[3] = {chi.x - clo.x, chi.y - clo.y, chi.z - clo.z};
+        const float cl[3] = {clo.x, clo.y, clo.z};
+        float best = FLT_MAX;
+        int best_axis = -1, best_bin = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!(ext[ax] > 0)) continue;
+            uint32_t cnt[16] = {0};
+            V3 blo[16], bhi[16];
+            for (int k = 0; k < 16; ++k) { blo[k] = mk(FLT_MAX, FLT_MAX, FLT_MAX); bhi[k] = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX); }
+            for (uint32_t i = b; i < e; ++i) {
+                const float cv = ax == 0 ? prims[i].c.x : (ax == 1 ? prims[i].c.y : prims[i].c.z);
+                int k = (int)((cv - cl[ax]) / ext[ax] * 16.0f);
+                k = std::min(15, std::max(0, k));
+                ++cnt[k];
+                blo[k] = hmin(blo[k], prims[i].lo);
+                bhi[k] = hmax(bhi[k], prims[i].hi);
+            }
+            float rarea[16];
+            uint32_t rcnt[16];
+            V3 rlo = mk(FLT_MAX, FLT_MAX, FLT_MAX), rhi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            uint32_t rc = 0;
+            for (int k = 15; k >= 1; --k) {
+                rlo = hmin(rlo, blo[k]); rhi = hmax(rhi, bhi[k]); rc += cnt[k];
+                rarea[k] = rc ? true_area(rlo, rhi) : 0.0f;
+                rcnt[k] = rc;
+            }
+            V3 llo = mk(FLT_MAX, FLT_MAX, FLT_MAX), lhi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            uint32_t lc = 0;
+            for (int k = 0; k < 15; ++k) {
+                llo = hmin(llo, blo[k]); lhi = hmax(lhi, bhi[k]); lc += cnt[k];
+                if (lc == 0 || rcnt[k + 1] == 0) continue;
+                const float c = true_area(llo, lhi) * (float)lc + rarea[k + 1] * (float)rcnt[k + 1];
+                if (c < best) { best = c; best_axis = ax; best_bin = k; }
+            }
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = b + (e - b) / 2;  // coincident centroids: split by index
+        } else {
+            auto it = std::partition(prims.begin() + b, prims.begin() + e, [&](const CullPrim& p) {
+                const float cv = best_axis == 0 ? p.c.x : (best_axis == 1 ? p.c.y : p.c.z);
+                int k = (int)((cv - cl[best_axis]) / ext[best_axis] * 16.0f);
+                k = std::min(15, std::max(0, k));
+                return k <= best_bin;
+            });
+            mid = (uint32_t)(it - prims.begin());
+            if (mid == b || mid == e) mid = b + (e - b) / 2;
+        }
+        const int l = build(b, mid, depth + 1, max_depth);
+        const int r = build(mid, e, depth + 1, max_depth);
+        pool[id].child = l;
+        pool[id].right = r;
+        return id;
+    }
+};
+
+void emit_cull(const std::vector<CullNode>& pool, const std::vector<CullPrim>& prims, const FlatBVH& ref,
+               int id, CullBVH& out)
+{
+    const CullNode& n = pool[id];
+    const uint32_t index = out.n_nodes++;
+    uint32_t w[8];
+    std::memcpy(&w[0], &n.lo.x, 4); std::memcpy(&w[1], &n.lo.y, 4); std::memcpy(&w[2], &n.lo.z, 4);
+    std::memcpy(&w[4], &n.hi.x, 4); std::memcpy(&w[5], &n.hi.y, 4); std::memcpy(&w[6], &n.hi.z, 4);
+    w[3] = 0;
+    w[7] = 0;
+    if (n.child < 0) {
+        const uint32_t* rw = &ref.nodes[8 * (size_t)prims[n.prim].ref];
+        w[3] = rw[3];  // first soup triangle
+        w[7] = rw[7];  // kLeafFlag | count
+    }
+    out.nodes.insert(out.nodes.end(), w, w + 8);
+    out.ref_leaf.push_back(n.child < 0 ? prims[n.prim].ref : 0xFFFFFFFFu);
+    if (n.child >= 0) {
+        emit_cull(pool, prims, ref, n.child, out);
+        emit_cull(pool, prims, ref, n.right, out);
+        out.nodes[8 * (size_t)index + 7] = out.n_nodes;  // jump past the subtree
+    }
+}
+
+}  // namespace
+
+void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float pad, CullBVH& out)
+{
+    std::vector<CullPrim> prims;
+    prims.reserve(ref.n_leaves);
+    for (uint32_t i = 0; i < ref.n_nodes; ++i) {
+        const uint32_t* w = &ref.nodes[8 * (size_t)i];
+        if (!(w[7] & kLeafFlag)) continue;
+        V3 lo = mk(fbits(w[0]), fbits(w[1]), fbits(w[2])), hi = mk(fbits(w[4]), fbits(w[5]), fbits(w[6]));
+        const uint32_t first = w[3], cnt = w[7] & ~kLeafFlag;
+        for (uint32_t k = 3 * first; k < 3 * (first + cnt); ++k) {
+            const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
+            lo = hmin(lo, p);
+            hi = hmax(hi, p);
+        }
+        CullPrim cp;
+        cp.lo = lo - mk(pad, pad, pad);
+        cp.hi = hi + mk(pad, pad, pad);
+        cp.c = (cp.lo + cp.hi) * 0.5f;
+        cp.ref = i;
+        prims.push_back(cp);
+    }
+    out = CullBVH();
+    if (prims.empty()) return;
+    CullBuilder cb{prims, {}};
+    cb.pool.reserve(2 * prims.size());
+    cb.build(0, (uint32_t)prims.size(), 0, out.depth);
+    // DFS pre-order with skip links, in the node format of the reference layout
+    out.nodes.reserve(8 * cb.pool.size());
+    out.ref_leaf.reserve(cb.pool.size());
+    emit_cull(cb.pool, prims, ref, 0, out);
+}
+
 }  // namespace fornos
 
 extern "C" fornos_status fornos_bvh_reference_export(const fornos_mesh* m, uint32_t max_tris,

@@ -58,4 +58,18 @@ constexpr uint32_t kLeafFlag = 0x80000000u;
 void build_reference_bvh(const fornos_mesh& m, uint32_t max_tris, uint32_t max_depth, FlatBVH& out);
 void export_reference_nodes(const FlatBVH& B, uint32_t* nodes9);  // BVHGPUData, 9 words per node
 
+// Parent of every reference node (root: 0xFFFFFFFF), from the skip links.
+std::vector<uint32_t> reference_parents(const FlatBVH& B);
+
+// Culling tree over the reference leaves: a binned-SAH binary tree whose
+// leaves are the reference leaves (same triangles), each box the union of
+// the reference leaf box and its triangles, padded by `pad`.  Same 8-word node
+// format as FlatBVH; ref_leaf[i] is the reference node of culling leaf i.
+struct CullBVH {
+    std::vector<uint32_t> nodes;
+    std::vector<uint32_t> ref_leaf;
+    uint32_t n_nodes = 0, depth = 0;
+};
+void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& soup_pos4, float pad, CullBVH& out);
+
 }  // namespace fornos

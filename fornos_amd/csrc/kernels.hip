@@ -42,11 +42,14 @@ constexpr uint32_t kGroup = 64;                // k_groupSize, Src/solver_ao.cpp
 constexpr int kBlock = 256;
 
 struct DevScene {
-    const float4* __restrict__ nodes;
+    const float4* __restrict__ nodes;    // reference tree
     const float4* __restrict__ tri;
     const float4* __restrict__ spos;
     const float4* __restrict__ snrm;
-    uint32_t n_nodes;
+    const uint32_t* __restrict__ parent;  // reference tree parent links
+    const float4* __restrict__ cnodes;   // culling tree over the reference leaves
+    const uint32_t* __restrict__ cref;   // reference leaf of each culling leaf
+    uint32_t n_nodes, n_cnodes;
 };
 
 __device__ inline V3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
@@ -203,6 +206,34 @@ __device__ inline bool node_visit(const Ray& r, float4 lo, float4 hi, float T)
     return ray_aabb(r.o, r.d, lo, hi) < T;
 }
 
+// Conservative form of the same test for the culling tree: false only when
+// the box is missed for certain (its boxes are padded, so certainty suffices).
+__device__ inline bool node_maybe(const Ray& r, float4 lo, float4 hi, float T)
+{
+    const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
+    const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
+    const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
+    const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+    const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+    const float al = a - __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
+    const float bh = b + __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
+    return !(bh < 0.0f || al > bh || al >= T);
+}
+
+// AO and BN decide a ray on its first qualifying hit (ao_step1.comp:158,
+// bentnormals_step1.comp:170), and the reference finds that hit iff the
+// triangle's leaf is reachable: every node from the root down to it has slab
+// entry < maxD (before the first hit the running minimum is FLT_MAX, and once
+// a hit below maxD exists the ray is decided).  So a hit found through the
+// culling tree counts exactly when its reference leaf's chain passes the
+// reference test.
+__device__ inline bool chain_ok(const Ray& r, const DevScene& sc, uint32_t leaf, float T)
+{
+    for (uint32_t x = leaf; x != kMiss; x = sc.parent[x])
+        if (!node_visit(r, sc.nodes[2 * x], sc.nodes[2 * x + 1], T)) return false;
+    return true;
+}
+
 // Shaders/ao_step1.comp:109-147 with the per-triangle terms read from `tri`.
 // Returns t, or FLT_MAX.  Triangles whose t falls outside [tmin, tmax) cannot
 // change the caller's result and are rejected before the barycentrics.
@@ -281,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
     ray.slow = false;
     uint32_t rid = 0, node = 0;
     float best = FLT_MAX;
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, ref_path = false;
     uint32_t cn = 0, ct = 0, nr = 0;
     for (;;) {
         const uint64_t idle = __ballot(!active);
@@ -307,6 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
                         ray.inv = mk(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
                         ray.slow = !(fabsf(ray.d.x) >= kSlowDir && fabsf(ray.d.y) >= kSlowDir &&
                                      fabsf(ray.d.z) >= kSlowDir);
+                        ref_path = ray.slow;  // tiny direction components: reference tree, exact test
                         rid = r;
                         node = 0;
                         best = FLT_MAX;
@@ -317,7 +349,56 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
             }
         }
         if (exhausted && __ballot(active) == 0) break;
-        if (active) {
+        if (active && !ref_path) {
+            // The culling tree.  AO / BN: a qualifying hit decides the ray iff
+            // its reference leaf is reachable (chain_ok with maxD).
+            // Thickness: a hit at t < best is taken when its leaf's chain
+            // passes with min(maxD, t): then the reference, whose running
+            // minimum never drops below the final one, visits that leaf too,
+            // and the minimum over reachable leaves is the reference's value.
+            // A leaf reachable with maxD but not with min(maxD, t) (a box
+            // entered no earlier than the hit) sends the ray to the reference
+            // tree instead.
+            const float4 lo = A.sc.cnodes[2 * node], hi = A.sc.cnodes[2 * node + 1];
+            const uint32_t w = __float_as_uint(hi.w);
+            const float T = KIND == kThick ? fminf(best, A.maxd) : A.maxd;
+            bool found = false, fallback = false;
+            if (COUNT) ++cn;
+            if (node_maybe(ray, lo, hi, T)) {
+                if (w & kLeafFlag) {
+                    const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
+                    const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : best);
+                    for (uint32_t k = first; k < end; ++k) {
+                        if (COUNT) ++ct;
+                        const float t = hemi_tri(ray.o, ray.d, k, A.sc, A.mind, KIND == kThick ? best : tri_hi);
+                        if (t == FLT_MAX) continue;
+                        const uint32_t leaf = A.sc.cref[node];
+                        if (KIND != kThick) {
+                            found = chain_ok(ray, A.sc, leaf, A.maxd);
+                            break;  // the other triangles of the leaf share its chain
+                        }
+                        if (chain_ok(ray, A.sc, leaf, fminf(A.maxd, t))) {
+                            best = t;
+                        } else {
+                            fallback = chain_ok(ray, A.sc, leaf, A.maxd);
+                            break;  // unreachable leaf: none of its triangles count
+                        }
+                    }
+                }
+                ++node;
+            } else {
+                node = (w & kLeafFlag) ? node + 1 : w;
+            }
+            if (fallback) {
+                ref_path = true;  // redo this ray on the reference tree
+                node = 0;
+                best = FLT_MAX;
+            } else if (found || node >= A.sc.n_cnodes) {
+                if (KIND == kThick) A.tval[rid] = best;
+                else A.hit[rid] = found ? 1 : 0;
+                active = false;
+            }
+        } else if (active) {
             const float4 lo = A.sc.nodes[2 * node], hi = A.sc.nodes[2 * node + 1];
             const uint32_t w = __float_as_uint(hi.w);
             const float T = KIND == kThick ? fminf(best, A.maxd) : A.maxd;
@@ -557,13 +638,15 @@ struct fornos_scene {
     uint32_t persistent_blocks = 2048;  // k_rays grid: 8 blocks of 256 lanes per CU
     hipStream_t stream = nullptr;
     DevScene d{};
-    float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr;
+    float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr, *cnodes = nullptr;
+    uint32_t *parent = nullptr, *cref = nullptr;
     FlatBVH bvh;
     fornos_scene_info info{};
     ~fornos_scene()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
+        dev_free(cnodes); dev_free(parent); dev_free(cref);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -752,9 +835,30 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
         tri[4 * k + 2] = make_float4(v0.x, v0.y, v0.z, dot(v0, v1));
         tri[4 * k + 3] = make_float4(v1.x, v1.y, v1.z, dot(v1, v1));
     }
+    // culling tree over the reference leaves, boxes padded by 2^-16 of the
+    // largest coordinate magnitude (far above the predicate's rounding)
+    float amax = 1.0f;
+    for (const float4& q : spos) amax = std::max(amax, std::max(fabsf(q.x), std::max(fabsf(q.y), fabsf(q.z))));
+    CullBVH cull;
+    {
+        std::vector<float> sp4((size_t)4 * spos.size());
+        std::memcpy(sp4.data(), spos.data(), sizeof(float) * sp4.size());
+        build_cull_bvh(B, sp4, amax * (1.0f / 65536.0f), cull);
+    }
+    const std::vector<uint32_t> parent = reference_parents(B);
     const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
 
     HIP_TRY(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+    if (fornos_status st = dev_alloc(&sc->cnodes, (size_t)2 * cull.n_nodes)) return st;
+    if (fornos_status st = dev_alloc(&sc->cref, cull.ref_leaf.size())) return st;
+    if (fornos_status st = dev_alloc(&sc->parent, parent.size())) return st;
+    HIP_TRY(hipMemcpy(sc->cnodes, cull.nodes.data(), sizeof(uint32_t) * 8 * (size_t)cull.n_nodes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->cref, cull.ref_leaf.data(), sizeof(uint32_t) * cull.ref_leaf.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->parent, parent.data(), sizeof(uint32_t) * parent.size(), hipMemcpyHostToDevice));
+    sc->d.cnodes = sc->cnodes;
+    sc->d.cref = sc->cref;
+    sc->d.parent = sc->parent;
+    sc->d.n_cnodes = cull.n_nodes;
     if (fornos_status st = dev_alloc(&sc->nodes, (size_t)2 * B.n_nodes)) return st;
     if (fornos_status st = dev_alloc(&sc->tri, tri.size())) return st;
     if (fornos_status st = dev_alloc(&sc->spos, spos.size())) return st;
@@ -774,7 +878,9 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     sc->info.max_leaf_tris = B.max_leaf;
     sc->info.depth = B.depth;
     sc->info.build_seconds = build_s;
-    sc->info.device_bytes = sizeof(float4) * (2 * (size_t)B.n_nodes + tri.size() + spos.size() + snrm.size());
+    sc->info.device_bytes = sizeof(float4) * (2 * (size_t)B.n_nodes + tri.size() + spos.size() + snrm.size() +
+                                              2 * (size_t)cull.n_nodes) +
+                            sizeof(uint32_t) * (parent.size() + cull.ref_leaf.size());
     *out = sc.release();
     return FORNOS_OK;
 }
