@@ -98,6 +98,7 @@ _SIGS = [
     ("fornos_solver_reset", None, [H]),
     ("fornos_solver_read", None, [H, P, C.c_size_t]),
     ("fornos_solver_read_counts", None, [H, P]),
+    ("fornos_solver_last_timing", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_float), u32p]),
     ("fornos_solver_count_work", None, [H, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("fornos_solver_destroy", "void", [H]),
 ]

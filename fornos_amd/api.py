@@ -274,6 +274,12 @@ class Solver(_Handle):
         check(lib().fornos_solver_read_counts(self._h, ptr(out)))
         return out
 
+    def last_timing(self):
+        """(k_rays ms, k_reduce ms, k_rays launches) of the last run_all."""
+        a, b, n = C.c_float(), C.c_float(), C.c_uint32()
+        check(lib().fornos_solver_last_timing(self._h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
     def count_work(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c)))

@@ -43,7 +43,8 @@ constexpr int kBlock = 256;
 
 struct DevScene {
     const float4* __restrict__ nodes;    // reference tree
-    const float4* __restrict__ tri;
+    const float4* __restrict__ tri;      // (n, a.n) per triangle
+    const float4* __restrict__ trest;    // (a, d00), (v0, d01), (v1, d11) per triangle
     const float4* __restrict__ spos;
     const float4* __restrict__ snrm;
     const uint32_t* __restrict__ parent;  // reference tree parent links
@@ -74,7 +75,7 @@ template <int MODE>
 __device__ inline bool map_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmax, float& t_out,
                                V3& bc)
 {
-    const float4 q0 = sc.tri[4 * k];
+    const float4 q0 = sc.tri[k];
     const V3 n = xyz(q0);
     const float nd = dot(d, n);
     const bool side = MODE == 0 ? (fabsf(nd) > 0) : (MODE == 1 ? (nd > 0) : (nd < 0));
@@ -237,16 +238,16 @@ __device__ inline bool chain_ok(const Ray& r, const DevScene& sc, uint32_t leaf,
 // Shaders/ao_step1.comp:109-147 with the per-triangle terms read from `tri`.
 // Returns t, or FLT_MAX.  Triangles whose t falls outside [tmin, tmax) cannot
 // change the caller's result and are rejected before the barycentrics.
-__device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmin, float tmax)
+__device__ inline float hemi_tri_q(V3 o, V3 d, float4 q0, uint32_t k, const DevScene& sc, float tmin,
+                                   float tmax)
 {
-    const float4 q0 = sc.tri[4 * k];
     const V3 n = xyz(q0);
     const float nd = dot(d, n);
     if (!(fabsf(nd) > 0)) return FLT_MAX;
     const float pn = dot(o, n);
     const float t = (q0.w - pn) / nd;
     if (!(t >= 0 && t >= tmin && t < tmax)) return FLT_MAX;
-    const float4 q1 = sc.tri[4 * k + 1], q2 = sc.tri[4 * k + 2], q3 = sc.tri[4 * k + 3];
+    const float4 q1 = sc.trest[3 * k], q2 = sc.trest[3 * k + 1], q3 = sc.trest[3 * k + 2];
     const V3 p = o + d * t;
     const V3 v2 = p - xyz(q1), v0 = xyz(q2), v1 = xyz(q3);
     const float d00 = q1.w, d01 = q2.w, d11 = q3.w;
@@ -256,6 +257,11 @@ __device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, flo
     const float z = (d00 * d21 - d01 * d20) / denom;
     const float x = 1.0f - y - z;
     return (x >= 0 && y >= 0 && y <= 1 && z >= 0 && z <= 1) ? t : FLT_MAX;
+}
+
+__device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmin, float tmax)
+{
+    return hemi_tri_q(o, d, sc.tri[k], k, sc, tmin, tmax);
 }
 
 struct Basis {
@@ -367,21 +373,31 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
             if (node_maybe(ray, lo, hi, T)) {
                 if (w & kLeafFlag) {
                     const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
-                    const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : best);
-                    for (uint32_t k = first; k < end; ++k) {
-                        if (COUNT) ++ct;
-                        const float t = hemi_tri(ray.o, ray.d, k, A.sc, A.mind, KIND == kThick ? best : tri_hi);
-                        if (t == FLT_MAX) continue;
-                        const uint32_t leaf = A.sc.cref[node];
-                        if (KIND != kThick) {
-                            found = chain_ok(ray, A.sc, leaf, A.maxd);
-                            break;  // the other triangles of the leaf share its chain
-                        }
-                        if (chain_ok(ray, A.sc, leaf, fminf(A.maxd, t))) {
-                            best = t;
-                        } else {
-                            fallback = chain_ok(ray, A.sc, leaf, A.maxd);
-                            break;  // unreachable leaf: none of its triangles count
+                    const float tri_hi = KIND == kAO ? A.maxd : INFINITY;
+                    bool stop = false;
+                    for (uint32_t k0 = first; k0 < end && !stop; k0 += 4) {
+                        // the leaf's (n, a.n) records are contiguous: fetch four at once
+                        float4 q[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) q[j] = k0 + j < end ? A.sc.tri[k0 + j] : make_float4(0, 0, 0, 0);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t k = k0 + j;
+                            if (stop || k >= end) continue;
+                            if (COUNT) ++ct;
+                            const float t = hemi_tri_q(ray.o, ray.d, q[j], k, A.sc, A.mind,
+                                                       KIND == kThick ? best : tri_hi);
+                            if (t == FLT_MAX) continue;
+                            const uint32_t leaf = A.sc.cref[node];
+                            if (KIND != kThick) {
+                                found = chain_ok(ray, A.sc, leaf, A.maxd);
+                                stop = true;  // the other triangles of the leaf share its chain
+                            } else if (chain_ok(ray, A.sc, leaf, fminf(A.maxd, t))) {
+                                best = t;
+                            } else {
+                                fallback = chain_ok(ray, A.sc, leaf, A.maxd);
+                                stop = true;  // unreachable leaf: none of its triangles count
+                            }
                         }
                     }
                 }
@@ -679,11 +695,17 @@ struct fornos_solver {
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t* counter = nullptr;       // k_rays work counter
+    // per-launch events of the last timed run: (before k_rays, after k_rays, after k_reduce)
+    std::vector<hipEvent_t> events;
+    bool timed = false;
+    float rays_ms = 0, reduce_ms = 0;
+    uint32_t launches = 0;
     ~fornos_solver()
     {
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
         dev_free(samples); dev_free(out); dev_free(counts); dev_free(stats);
         dev_free(raybuf); dev_free(counter);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
     bool hemi() const { return kind >= FORNOS_SOLVER_AO; }
     uint32_t comps() const
@@ -735,11 +757,20 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         A.ntex = R.ntex = nt;
         const uint32_t blocks = std::min<uint32_t>(sc->persistent_blocks, div_up((uint64_t)nt * S, kBlock));
         hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc->stream, s->counter, 1u);
+        hipEvent_t* ev = nullptr;
+        if (s->timed) {
+            s->events.resize(s->events.size() + 3, nullptr);
+            ev = &s->events[s->events.size() - 3];
+            for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+            HIP_TRY(hipEventRecord(ev[0], sc->stream));
+        }
         if (count)
             hipLaunchKernelGGL((k_rays<KIND, true>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
         else
             hipLaunchKernelGGL((k_rays<KIND, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
+        if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
+        if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
         HIP_TRY(hipGetLastError());
     }
     return FORNOS_OK;
@@ -830,10 +861,10 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
         }
         const V3 v0 = p[1] - p[0], v1 = p[2] - p[0];
         const V3 nn = normalize(cross(v0, v1));
-        tri[4 * k + 0] = make_float4(nn.x, nn.y, nn.z, dot(p[0], nn));
-        tri[4 * k + 1] = make_float4(p[0].x, p[0].y, p[0].z, dot(v0, v0));
-        tri[4 * k + 2] = make_float4(v0.x, v0.y, v0.z, dot(v0, v1));
-        tri[4 * k + 3] = make_float4(v1.x, v1.y, v1.z, dot(v1, v1));
+        tri[k] = make_float4(nn.x, nn.y, nn.z, dot(p[0], nn));
+        tri[nt + 3 * (size_t)k + 0] = make_float4(p[0].x, p[0].y, p[0].z, dot(v0, v0));
+        tri[nt + 3 * (size_t)k + 1] = make_float4(v0.x, v0.y, v0.z, dot(v0, v1));
+        tri[nt + 3 * (size_t)k + 2] = make_float4(v1.x, v1.y, v1.z, dot(v1, v1));
     }
     // culling tree over the reference leaves, boxes padded by 2^-16 of the
     // largest coordinate magnitude (far above the predicate's rounding)
@@ -869,6 +900,7 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     HIP_TRY(hipMemcpy(sc->snrm, snrm.data(), sizeof(float4) * snrm.size(), hipMemcpyHostToDevice));
     sc->d.nodes = sc->nodes;
     sc->d.tri = sc->tri;
+    sc->d.trest = sc->tri + nt;
     sc->d.spos = sc->spos;
     sc->d.snrm = sc->snrm;
     sc->d.n_nodes = B.n_nodes;
@@ -1090,9 +1122,33 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
     const uint32_t t0 = (uint32_t)std::min<uint64_t>(s->offset / S, n);
     Timer tm(s->mm->sc->stream);
     if (fornos_status st = tm.start()) return st;
-    if (fornos_status st = solver_launch(s, t0, n, false)) return st;
+    for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+    s->events.clear();
+    s->timed = s->hemi();
+    const fornos_status st = solver_launch(s, t0, n, false);
+    s->timed = false;
+    if (st) return st;
     s->offset = s->work_total;
-    return tm.stop(ms);
+    if (fornos_status st2 = tm.stop(ms)) return st2;
+    s->rays_ms = s->reduce_ms = 0;
+    s->launches = (uint32_t)(s->events.size() / 3);
+    for (size_t i = 0; i + 2 < s->events.size(); i += 3) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, s->events[i], s->events[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, s->events[i + 1], s->events[i + 2]));
+        s->rays_ms += a;
+        s->reduce_ms += b;
+    }
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_last_timing(const fornos_solver* s, float* rays_ms, float* reduce_ms, uint32_t* launches)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_last_timing: null handle");
+    if (rays_ms) *rays_ms = s->rays_ms;
+    if (reduce_ms) *reduce_ms = s->reduce_ms;
+    if (launches) *launches = s->launches;
+    return FORNOS_OK;
 }
 
 fornos_status fornos_solver_progress(const fornos_solver* s, float* out)

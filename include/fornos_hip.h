@@ -184,6 +184,11 @@ fornos_status fornos_solver_read(const fornos_solver *s, void *out, size_t bytes
 /* Per-texel ray counts of the last hemisphere run: occluded rays (AO, BN) or
  * rays that hit (thickness).  count uint32 entries. */
 fornos_status fornos_solver_read_counts(const fornos_solver *s, uint32_t *out);
+/* Kernel time of the last run_all of a hemisphere solver, split into the
+ * ray launches (k_rays) and the per-texel reductions (k_reduce), from HIP
+ * events on the scene's stream; *launches = number of k_rays launches. */
+fornos_status fornos_solver_last_timing(const fornos_solver *s, float *rays_ms, float *reduce_ms,
+                                        uint32_t *launches);
 /* Traversal work of the last run_all of a hemisphere solver, counted by an
  * instrumented run of the same kernel: node tests, triangle tests, rays. */
 fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
