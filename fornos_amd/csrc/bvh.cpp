@@ -430,6 +430,65 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
     out.nodes.reserve(8 * cb.pool.size());
     out.ref_leaf.reserve(cb.pool.size());
     emit_cull(cb.pool, prims, ref, 0, out);
+    // 4-wide collapse: open the largest-area internal child until 4 children
+    struct Wide {
+        const std::vector<CullNode>& pool;
+        const std::vector<CullPrim>& prims;
+        const FlatBVH& ref;
+        CullBVH& out;
+        uint32_t emit(int id)
+        {
+            std::vector<int> ch;
+            const CullNode& n = pool[id];
+            if (n.child < 0) ch.push_back(id);
+            else { ch.push_back(n.child); ch.push_back(n.right); }
+            while (ch.size() < 4) {
+                int best = -1;
+                float ba = -1;
+                for (size_t i = 0; i < ch.size(); ++i) {
+                    const CullNode& c = pool[ch[i]];
+                    if (c.child < 0) continue;
+                    const float a = true_area(c.lo, c.hi);
+                    if (a > ba) { ba = a; best = (int)i; }
+                }
+                if (best < 0) break;
+                const int open = ch[best];
+                ch[best] = pool[open].child;
+                ch.insert(ch.begin() + best + 1, pool[open].right);
+            }
+            const uint32_t index = out.n_wide++;
+            out.wide.resize((size_t)32 * out.n_wide, 0);
+            uint32_t w[32];
+            for (int k = 0; k < 4; ++k) {
+                float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                uint32_t c = kNoChild;
+                if (k < (int)ch.size()) {
+                    const CullNode& cn = pool[ch[k]];
+                    lo[0] = cn.lo.x; lo[1] = cn.lo.y; lo[2] = cn.lo.z;
+                    hi[0] = cn.hi.x; hi[1] = cn.hi.y; hi[2] = cn.hi.z;
+                    if (cn.child < 0) {
+                        const uint32_t leaf = (uint32_t)(out.wide_leaves.size() / 4);
+                        const uint32_t* rw = &ref.nodes[8 * (size_t)prims[cn.prim].ref];
+                        const uint32_t lw[4] = {rw[3], rw[7] & ~kLeafFlag, prims[cn.prim].ref, 0};
+                        out.wide_leaves.insert(out.wide_leaves.end(), lw, lw + 4);
+                        c = kLeafFlag | leaf;
+                    } else {
+                        c = emit(ch[k]);
+                    }
+                }
+                for (int a = 0; a < 3; ++a) {
+                    std::memcpy(&w[4 * a + k], &lo[a], 4);
+                    std::memcpy(&w[12 + 4 * a + k], &hi[a], 4);
+                }
+                w[24 + k] = c;
+                w[28 + k] = 0;
+            }
+            std::memcpy(&out.wide[(size_t)32 * index], w, sizeof(w));
+            return index;
+        }
+    };
+    Wide wd{cb.pool, prims, ref, out};
+    wd.emit(0);
 }
 
 }  // namespace fornos

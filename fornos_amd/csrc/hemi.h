@@ -1,0 +1,384 @@
+// hemi.h -- hemisphere passes (AO, bent normals, thickness).
+//
+// k_rays runs persistent waves.  Each lane owns one ray: the ray basis of
+// Shaders/ao_step0.comp and one sample of ao/bentnormals/thick_step1.comp.
+// A lane whose ray has finished takes a new one from a global counter, one
+// atomic per wave for all of its idle lanes, so a wave never idles behind its
+// slowest ray.  Traversal is "while-while": every lane walks the 4-wide
+// culling tree until it holds a leaf (or is done), then the wave tests
+// triangles together, so the triangle code never runs for one lane while the
+// others wait on nodes.
+//
+// Exactness.  The culling trees are not the reference's tree, so a hit found
+// through them counts only under the reference's rules (chain_ok in
+// device.h): AO and BN take a qualifying hit iff its reference leaf is
+// reachable; thickness takes a hit at t < best iff its leaf's chain passes
+// with min(maxD, t), and otherwise (a box entered no earlier than the hit)
+// the ray is redone on the reference tree.  Rays with a direction component
+// below 2^-40 use the reference tree and its exact test from the start, and a
+// ray whose 4-wide stack would overflow is redone on the stackless binary
+// culling tree.  Results therefore equal the reference's ray for ray.
+//
+// Only hits are written: AO adds 1 to its texel's counter, BN sets its
+// sample's bit, thickness stores t (the buffer starts at FLT_MAX).  k_reduce
+// then folds each texel's S results in sample order (step2 + tangentspace),
+// the same split as the reference's step1/step2 dispatches.
+#pragma once
+#include "device.h"
+
+namespace fornos {
+namespace {
+
+enum { kAO = 0, kBN = 1, kThick = 2 };
+enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
+constexpr uint32_t kRefill = 16;  // refill a wave once this many lanes are idle
+constexpr uint32_t kStack = 24;   // 4-wide traversal stack entries per lane
+
+struct RayArgs {
+    DevScene sc;
+    const float4* __restrict__ coords;
+    const uint32_t* __restrict__ tidx;
+    const float4* __restrict__ samples;
+    uint32_t* __restrict__ cnt;         // AO: occluded rays per texel (global texel index)
+    uint32_t* __restrict__ mask;        // BN: hit bits, mwords words per texel of the launch
+    float* __restrict__ tval;           // thickness: closest t per ray of the launch
+    unsigned int* __restrict__ counter;  // next ray of the launch to hand out
+    unsigned long long* __restrict__ stats;  // box tests, triangle tests, rays (COUNT only)
+    uint32_t S, P, t0, ntex, mwords;    // texels [t0, t0 + ntex), S rays each
+    float mind, maxd;
+};
+
+struct Lane {
+    Ray ray;
+    uint32_t q, s;      // texel of the launch, sample
+    uint32_t cur, sp;   // 4-wide tree: next item (node, kLeafFlag | leaf, kNoChild = pop)
+    uint32_t node;      // binary trees: next node
+    float best;         // thickness: closest accepted hit
+    int mode;
+    bool active;
+};
+
+template <int KIND>
+__device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
+{
+    if (KIND == kAO) {
+        if (hit) atomicAdd(&A.cnt[A.t0 + L.q], 1u);
+    } else if (KIND == kBN) {
+        if (hit) atomicOr(&A.mask[(size_t)L.q * A.mwords + (L.s >> 5)], 1u << (L.s & 31));
+    } else {
+        if (L.best != FLT_MAX) A.tval[(size_t)L.q * A.S + L.s] = L.best;
+    }
+    L.active = false;
+}
+
+// Triangles [first, first + count) of a reference leaf whose node is `leaf`,
+// under the hit rules above.  Returns 1 when the ray is decided (AO/BN hit),
+// 2 when it must be redone on the reference tree, 0 otherwise.
+template <int KIND, bool COUNT>
+__device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint32_t count, uint32_t leaf,
+                                uint32_t& ct)
+{
+    const uint32_t end = first + count;
+    const float tri_hi = KIND == kAO ? A.maxd : INFINITY;
+    for (uint32_t k0 = first; k0 < end; k0 += 4) {
+        // the leaf's (n, a.n) records are contiguous: fetch four at once
+        float4 q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = k0 + j < end ? A.sc.tri[k0 + j] : make_float4(0, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t k = k0 + j;
+            if (k >= end) continue;
+            if (COUNT) ++ct;
+            const float t = hemi_tri_q(L.ray.o, L.ray.d, q[j], k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi);
+            if (t == FLT_MAX) continue;
+            if (KIND != kThick) return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 1 : 0;  // one chain per leaf
+            if (chain_ok(L.ray, A.sc, leaf, fminf(A.maxd, t))) L.best = t;
+            else return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 2 : 0;
+        }
+    }
+    return 0;
+}
+
+// One node of the 4-wide culling tree (or one stack pop).
+template <int KIND, bool COUNT>
+__device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, float* stk_t, uint32_t& cn)
+{
+    const uint32_t tid = threadIdx.x;
+    const float T = KIND == kThick ? fminf(L.best, A.maxd) : A.maxd;
+    if (L.cur == kNoChild) {
+        if (L.sp == 0) {
+            finish<KIND>(A, L, false);
+            return;
+        }
+        --L.sp;
+        L.cur = stk_node[L.sp * kBlock + tid];
+        // thickness: a stacked subtree entered beyond the best hit is dead
+        if (KIND == kThick && stk_t[L.sp * kBlock + tid] >= T) L.cur = kNoChild;
+        return;
+    }
+    const float4* nd = A.sc.wide + 8 * (size_t)L.cur;
+    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+    const uint4 ch = *(const uint4*)(nd + 6);
+    const float4 lo[4] = {make_float4(lx.x, ly.x, lz.x, 0), make_float4(lx.y, ly.y, lz.y, 0),
+                          make_float4(lx.z, ly.z, lz.z, 0), make_float4(lx.w, ly.w, lz.w, 0)};
+    const float4 hi[4] = {make_float4(hx.x, hy.x, hz.x, 0), make_float4(hx.y, hy.y, hz.y, 0),
+                          make_float4(hx.z, hy.z, hz.z, 0), make_float4(hx.w, hy.w, hz.w, 0)};
+    const uint32_t chs[4] = {ch.x, ch.y, ch.z, ch.w};
+    float e[4];
+    uint32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (COUNT && chs[k] != kNoChild) ++cn;
+        float al = 0;
+        const bool pass = chs[k] != kNoChild && box_maybe(L.ray, lo[k], hi[k], T, al);
+        e[k] = pass ? al : INFINITY;
+        c[k] = pass ? chs[k] : kNoChild;
+    }
+    // nearest first: sort the four (entry, child) pairs
+#define FORNOS_CSWAP(i, j)                                            \
+    if (e[j] < e[i]) {                                                \
+        const float te = e[i]; e[i] = e[j]; e[j] = te;                \
+        const uint32_t tc = c[i]; c[i] = c[j]; c[j] = tc;             \
+    }
+    FORNOS_CSWAP(0, 1) FORNOS_CSWAP(2, 3) FORNOS_CSWAP(0, 2) FORNOS_CSWAP(1, 3) FORNOS_CSWAP(1, 2)
+#undef FORNOS_CSWAP
+    L.cur = c[0];
+#pragma unroll
+    for (int k = 3; k >= 1; --k) {
+        if (c[k] == kNoChild) continue;
+        if (L.sp == kStack) {  // redo the ray on the stackless binary culling tree
+            L.mode = kCullTree;
+            L.node = 0;
+            L.best = FLT_MAX;
+            return;
+        }
+        stk_node[L.sp * kBlock + tid] = c[k];
+        if (KIND == kThick) stk_t[L.sp * kBlock + tid] = e[k];
+        ++L.sp;
+    }
+}
+
+// One node of a binary tree in skip-link order: the culling tree (boxes
+// tested conservatively, hit rules above) or the reference tree (the
+// reference's own test and traversal, Shaders/ao_step1.comp:166-193).
+template <int KIND, bool COUNT>
+__device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint32_t& ct)
+{
+    const bool ref = L.mode == kRefTree;
+    const float4* nodes = ref ? A.sc.nodes : A.sc.cnodes;
+    const uint32_t n_nodes = ref ? A.sc.n_nodes : A.sc.n_cnodes;
+    const float4 lo = nodes[2 * L.node], hi = nodes[2 * L.node + 1];
+    const uint32_t w = __float_as_uint(hi.w);
+    const float T = KIND == kThick ? fminf(L.best, A.maxd) : A.maxd;
+    if (COUNT) ++cn;
+    const bool visit = ref ? node_visit(L.ray, lo, hi, T) : node_maybe(L.ray, lo, hi, T);
+    if (visit) {
+        if (w & kLeafFlag) {
+            const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
+            if (ref) {
+                const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : L.best);
+                for (uint32_t k = first; k < end; ++k) {
+                    if (COUNT) ++ct;
+                    const float t = hemi_tri(L.ray.o, L.ray.d, k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi);
+                    if (t == FLT_MAX) continue;
+                    if (KIND != kThick) {
+                        finish<KIND>(A, L, true);
+                        return;
+                    }
+                    L.best = t;
+                }
+            } else {
+                const int r = leaf_hits<KIND, COUNT>(A, L, first, w & ~kLeafFlag, A.sc.cref[L.node], ct);
+                if (r == 1) {
+                    finish<KIND>(A, L, true);
+                    return;
+                }
+                if (r == 2) {
+                    L.mode = kRefTree;
+                    L.node = 0;
+                    L.best = FLT_MAX;
+                    return;
+                }
+            }
+        }
+        ++L.node;
+    } else {
+        L.node = (w & kLeafFlag) ? L.node + 1 : w;
+    }
+    if (L.node >= n_nodes) finish<KIND>(A, L, false);
+}
+
+template <int KIND, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
+{
+    __shared__ uint32_t stk_node[kStack * kBlock];
+    __shared__ float stk_t[KIND == kThick ? kStack * kBlock : 1];
+    const uint32_t total = A.ntex * A.S;
+    const uint32_t lane = threadIdx.x & 63;
+    Lane L;
+    L.ray.o = L.ray.d = L.ray.inv = mk(0, 0, 0);
+    L.ray.slow = false;
+    L.q = L.s = L.cur = L.sp = L.node = 0;
+    L.best = FLT_MAX;
+    L.mode = kWideTree;
+    L.active = false;
+    bool exhausted = false;
+    uint32_t cn = 0, ct = 0, nr = 0;
+    for (;;) {
+        // refill idle lanes, one atomic per wave
+        const uint64_t idle = __ballot(!L.active);
+        const uint32_t nidle = __popcll(idle);
+        if (!exhausted && (nidle >= kRefill || nidle == 64)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(A.counter, nidle);
+            base = __shfl(base, 0);
+            if (base + nidle >= total) exhausted = true;
+            if (!L.active) {
+                const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+                if (r < total) {
+                    const uint32_t q = r / A.S, s = r - q * A.S, g = A.t0 + q;
+                    const uint32_t t = A.tidx[g];
+                    if (t != kMiss) {
+                        const Basis B = ray_basis(A.sc, A.coords[g], t);
+                        const V3 d = KIND == kThick ? -B.d : B.d;
+                        L.ray.o = B.o;
+                        L.ray.d = sample_dir(B, d, A.samples[(size_t)(g % A.P) * A.S + s]);
+                        L.ray.inv = mk(1.0f / L.ray.d.x, 1.0f / L.ray.d.y, 1.0f / L.ray.d.z);
+                        L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
+                                       fabsf(L.ray.d.z) >= kSlowDir);
+                        L.mode = L.ray.slow ? kRefTree : kWideTree;
+                        L.q = q;
+                        L.s = s;
+                        L.cur = 0;
+                        L.sp = 0;
+                        L.node = 0;
+                        L.best = FLT_MAX;
+                        L.active = true;
+                        if (COUNT) ++nr;
+                    }
+                }
+            }
+        }
+        if (exhausted && __ballot(L.active) == 0) break;
+        // phase 1: walk the 4-wide tree until every lane holds a leaf or is done
+        for (;;) {
+            const bool walk = L.active && L.mode == kWideTree && (L.cur == kNoChild || !(L.cur & kLeafFlag));
+            if (__ballot(walk) == 0) break;
+            if (walk) wide_step<KIND, COUNT>(A, L, stk_node, stk_t, cn);
+        }
+        // phase 2: leaves together; rays on a binary tree take one step
+        if (L.active) {
+            if (L.mode == kWideTree) {
+                const uint4 lf = A.sc.wleaf[L.cur & ~kLeafFlag];
+                const int r = leaf_hits<KIND, COUNT>(A, L, lf.x, lf.y, lf.z, ct);
+                L.cur = kNoChild;
+                if (r == 1) {
+                    finish<KIND>(A, L, true);
+                } else if (r == 2) {
+                    L.mode = kRefTree;
+                    L.node = 0;
+                    L.best = FLT_MAX;
+                }
+            } else {
+                binary_step<KIND, COUNT>(A, L, cn, ct);
+            }
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = cn, b = ct, c = nr;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_down(a, off);
+            b += __shfl_down(b, off);
+            c += __shfl_down(c, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&A.stats[0], a);
+            atomicAdd(&A.stats[1], b);
+            atomicAdd(&A.stats[2], c);
+        }
+    }
+}
+
+struct ReduceArgs {
+    DevScene sc;
+    const float4* __restrict__ coords;
+    const uint32_t* __restrict__ tidx;
+    const float4* __restrict__ samples;
+    const float4* __restrict__ pixt;    // n, t, b per texel (BN tangent space) or null
+    const uint32_t* __restrict__ mask;
+    const float* __restrict__ tval;
+    float* __restrict__ out;            // 1 or 3 floats per texel (global texel index)
+    uint32_t* __restrict__ counts;      // per texel (AO: already counted by k_rays)
+    uint32_t S, P, t0, ntex, mwords;
+    float maxd;
+    int tangent;
+};
+
+// step2 reductions in sample order (ao_step2.comp:24-29, thick_step2.comp:24-29,
+// bentnormals_step2.comp:25-34) and tangentspace.comp:22-41.  Texels whose
+// mapping missed get AO 1, thickness maxD and bent normal (0,0,0).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= A.ntex) return;
+    const uint32_t g = A.t0 + q, S = A.S;
+    const uint32_t t = A.tidx[g];
+    if (KIND == kAO) {
+        // the reference's sequential float sum of 0/1 values is the exact count
+        A.out[g] = t == kMiss ? 1.0f : 1.0f - (float)A.counts[g] / (float)S;
+        return;
+    }
+    uint32_t cnt = 0;
+    if (KIND == kThick) {
+        float acc = 0;
+        if (t != kMiss) {
+            const float* tv = A.tval + (size_t)q * S;
+            for (uint32_t s = 0; s < S; ++s) {
+                const float v = tv[s];
+                if (v != FLT_MAX) ++cnt;
+                acc += v != FLT_MAX ? v : A.maxd;
+            }
+        }
+        A.out[g] = t == kMiss ? A.maxd : acc / (float)S;
+    } else {
+        V3 r = mk(0, 0, 0);
+        if (t != kMiss) {
+            const Basis B = ray_basis(A.sc, A.coords[g], t);
+            const float4* smp = A.samples + (size_t)(g % A.P) * S;
+            const uint32_t* m = A.mask + (size_t)q * A.mwords;
+            V3 acc = mk(0, 0, 0);
+            for (uint32_t s = 0; s < S; ++s) {
+                if ((m[s >> 5] >> (s & 31)) & 1u) {  // adds vec3(0): acc never holds -0
+                    ++cnt;
+                    continue;
+                }
+                acc = acc + sample_dir(B, B.d, smp[s]);
+            }
+            r = normalize(acc);
+            if (A.tangent) {
+                const V3 n = xyz(A.pixt[3 * g]), tt = xyz(A.pixt[3 * g + 1]), b = xyz(A.pixt[3 * g + 2]);
+                const V3 d0 = mk(n.z * b.y - n.y * b.z, n.x * b.z - n.z * b.x, n.y * b.x - n.x * b.y);
+                const V3 d1 = mk(tt.z * n.y - tt.y * n.z, tt.x * n.z - n.x * tt.z, n.x * tt.y - tt.x * n.y);
+                const V3 d2 = mk(tt.y * b.z - tt.z * b.y, tt.z * b.x - tt.x * b.z, tt.x * b.y - tt.y * b.x);
+                r = normalize(mk(dot(r, d0), dot(r, d1), dot(r, d2)));
+            }
+        }
+        A.out[3 * g] = r.x;
+        A.out[3 * g + 1] = r.y;
+        A.out[3 * g + 2] = r.z;
+    }
+    A.counts[g] = cnt;
+}
+
+// Fills n 32-bit words with v (a kernel rather than hipMemsetAsync, so the
+// whole pass can be captured into a HIP graph).
+__global__ void k_fill(uint32_t* p, size_t n, uint32_t v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+}  // namespace fornos

@@ -69,7 +69,14 @@ struct CullBVH {
     std::vector<uint32_t> nodes;
     std::vector<uint32_t> ref_leaf;
     uint32_t n_nodes = 0, depth = 0;
+    // The same tree collapsed to 4 children per node, 32 words per node:
+    // lo.x[4], lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4], child[4], pad[4].
+    // child: node index, kLeafFlag | leaf index, or kNoChild.  Leaves: 4 words
+    // {first soup triangle, count, reference leaf node, 0}.
+    std::vector<uint32_t> wide, wide_leaves;
+    uint32_t n_wide = 0;
 };
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& soup_pos4, float pad, CullBVH& out);
 
 }  // namespace fornos

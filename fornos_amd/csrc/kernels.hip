@@ -1,19 +1,8 @@
 // kernels.hip -- gfx950 kernels of the baking hot path and the device side of
 // the C ABI (include/fornos_hip.h).
 //
-// HBM layout of a scene (all soup-ordered, i.e. in the DFS leaf order of the
-// reference flatten, Src/meshmapping.cpp:99-116):
-//   nodes  2 x float4 per node: (min.xyz, first soup triangle),
-//                               (max.xyz, jump | kLeafFlag+count)
-//   tri    4 x float4 per triangle: the per-triangle terms of the hemisphere
-//          and mapping predicates, computed once on the host with the
-//          shader's own operation order:
-//          (n.xyz, dot(a,n)), (a.xyz, d00), (v0.xyz, d01), (v1.xyz, d11)
-//          with n = normalize(cross(b-a, c-a)), v0 = b-a, v1 = c-a
-//   spos   3 x float4 per triangle (reference Vector4 soup positions)
-//   snrm   3 x float4 per triangle (reference Vector4 soup normals)
-// Texel data: pix 2 x float4 per texel (p, d), pixt 3 x float4 (n, t, b),
-// coords float4 (t, b0, b1, b2), tidx uint (soup vertex index).
+// The kernels live in device.h (layout, predicates), mapping.h, hemi.h and
+// texel.h; this file holds the handles and the launches.
 //
 // Everything is compiled with -ffp-contract=off and IEEE-correct fp32
 // division/sqrt so that results are bit-identical to the CPU restatement.
@@ -28,552 +17,17 @@
 #include <string>
 #include <vector>
 
-#include "host.h"
+#include "device.h"
+#include "hemi.h"
+#include "mapping.h"
+#include "texel.h"
 
 #pragma clang fp contract(off)
 
 using namespace fornos;
 
+
 namespace {
-
-constexpr uint32_t kMiss = 0xFFFFFFFFu;
-constexpr uint32_t kWorkPerStep = 1024 * 128;  // k_workPerFrame, Src/solver_ao.cpp:33
-constexpr uint32_t kGroup = 64;                // k_groupSize, Src/solver_ao.cpp:32
-constexpr int kBlock = 256;
-
-struct DevScene {
-    const float4* __restrict__ nodes;    // reference tree
-    const float4* __restrict__ tri;      // (n, a.n) per triangle
-    const float4* __restrict__ trest;    // (a, d00), (v0, d01), (v1, d11) per triangle
-    const float4* __restrict__ spos;
-    const float4* __restrict__ snrm;
-    const uint32_t* __restrict__ parent;  // reference tree parent links
-    const float4* __restrict__ cnodes;   // culling tree over the reference leaves
-    const uint32_t* __restrict__ cref;   // reference leaf of each culling leaf
-    uint32_t n_nodes, n_cnodes;
-};
-
-__device__ inline V3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
-
-// RayAABB, Shaders/ao_step1.comp:43-52 (identical in every traversal shader).
-__device__ inline float ray_aabb(V3 o, V3 d, float4 lo, float4 hi)
-{
-    float t1x = (lo.x - o.x) / d.x, t1y = (lo.y - o.y) / d.y, t1z = (lo.z - o.z) / d.z;
-    float t2x = (hi.x - o.x) / d.x, t2y = (hi.y - o.y) / d.y, t2z = (hi.z - o.z) / d.z;
-    float a = gmax(gmin(t1x, t2x), gmax(gmin(t1y, t2y), gmin(t1z, t2z)));
-    float b = gmin(gmax(t1x, t2x), gmin(gmax(t1y, t2y), gmax(t1z, t2z)));
-    return (b >= 0 && a <= b) ? a : FLT_MAX;
-}
-
-// ---------------------------------------------------------------------------
-// Mesh mapping: Shaders/meshmapping.comp (MODE 0: |nd| > 0) and
-// meshmapping_nobackfaces.comp (MODE 1: forward nd > 0, MODE 2: backward
-// nd < 0).  t bound: t < tmax for every mode (the non-culling shader tests
-// t >= 0 and keeps strict-< minima, so skipping t >= current best is exact).
-// ---------------------------------------------------------------------------
-template <int MODE>
-__device__ inline bool map_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmax, float& t_out,
-                               V3& bc)
-{
-    const float4 q0 = sc.tri[k];
-    const V3 n = xyz(q0);
-    const float nd = dot(d, n);
-    const bool side = MODE == 0 ? (fabsf(nd) > 0) : (MODE == 1 ? (nd > 0) : (nd < 0));
-    if (!side) return false;
-    const float pn = dot(o, n);
-    const float t = (q0.w - pn) / nd;
-    if (!(t >= 0 && t < tmax)) return false;
-    const V3 pf = o + d * t;
-    const float4 a4 = sc.spos[3 * k], b4 = sc.spos[3 * k + 1], c4 = sc.spos[3 * k + 2];
-    // barycentric() in fp64, Shaders/meshmapping.comp:50-64
-    const double ax = a4.x, ay = a4.y, az = a4.z;
-    const double v0x = (double)b4.x - ax, v0y = (double)b4.y - ay, v0z = (double)b4.z - az;
-    const double v1x = (double)c4.x - ax, v1y = (double)c4.y - ay, v1z = (double)c4.z - az;
-    const double v2x = (double)pf.x - ax, v2y = (double)pf.y - ay, v2z = (double)pf.z - az;
-    const double d00 = v0x * v0x + v0y * v0y + v0z * v0z;
-    const double d01 = v0x * v1x + v0y * v1y + v0z * v1z;
-    const double d11 = v1x * v1x + v1y * v1y + v1z * v1z;
-    const double d20 = v2x * v0x + v2y * v0y + v2z * v0z;
-    const double d21 = v2x * v1x + v2y * v1y + v2z * v1z;
-    const double denom = d00 * d11 - d01 * d01;
-    const double y = (d11 * d20 - d01 * d21) / denom;
-    const double z = (d00 * d21 - d01 * d20) / denom;
-    const float bx = (float)(1.0 - y - z), by = (float)y, bz = (float)z;
-    const float lo = -1e-5f, hi = 1.0f;
-    if (bx >= lo && by >= lo && by <= hi && bz >= lo && bz <= hi) {
-        t_out = t;
-        bc = mk(bx, by, bz);
-        return true;
-    }
-    return false;
-}
-
-// One pass of raycastBVH / raycastBackBVH in the reference's stackless DFS
-// order (Shaders/meshmapping_nobackfaces.comp:102-121): visit a node iff its
-// slab entry is < the running best, else jump past its subtree.
-template <int MODE>
-__device__ inline void map_pass(V3 o, V3 d, const DevScene& sc, float& cur, uint32_t& idx, V3& bc)
-{
-    uint32_t i = 0;
-    while (i < sc.n_nodes) {
-        const float4 lo = sc.nodes[2 * i], hi = sc.nodes[2 * i + 1];
-        const uint32_t w = __float_as_uint(hi.w);
-        if (ray_aabb(o, d, lo, hi) < cur) {
-            if (w & kLeafFlag) {
-                const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
-                for (uint32_t k = first; k < end; ++k) {
-                    float t;
-                    V3 b;
-                    if (map_tri<MODE>(o, d, k, sc, cur, t, b)) { cur = t; idx = 3 * k; bc = b; }
-                }
-            }
-            ++i;
-        } else {
-            i = (w & kLeafFlag) ? i + 1 : w;
-        }
-    }
-}
-
-template <bool CULL>
-__global__ __launch_bounds__(kBlock) void k_meshmap(const float4* __restrict__ pix, uint32_t begin,
-                                                     uint32_t end, DevScene sc,
-                                                     float4* __restrict__ coords,
-                                                     uint32_t* __restrict__ tidx)
-{
-    const uint32_t g = begin + blockIdx.x * kBlock + threadIdx.x;
-    if (g >= end) return;
-    const V3 p = xyz(pix[2 * g]), d = xyz(pix[2 * g + 1]);
-    uint32_t idx = kMiss;
-    V3 bc = mk(0, 0, 0);
-    float t = FLT_MAX;
-    if (CULL) {
-        // meshmapping_nobackfaces.comp:182-199: one running t shared by both passes
-        map_pass<1>(p, d, sc, t, idx, bc);
-        map_pass<2>(p, -d, sc, t, idx, bc);
-    } else {
-        // meshmapping.comp:139-162: the backward pass starts from the forward t
-        map_pass<0>(p, d, sc, t, idx, bc);
-        map_pass<0>(p, -d, sc, t, idx, bc);
-    }
-    coords[g] = make_float4(t, bc.x, bc.y, bc.z);
-    tidx[g] = idx;
-}
-
-// ---------------------------------------------------------------------------
-// Hemisphere passes (AO, bent normals, thickness).
-//
-// k_rays: persistent waves.  Each lane owns one ray (ao_step0 ray basis +
-// ao/bentnormals/thick step1 traversal); a lane whose ray has finished takes
-// a new one from a global counter, one atomic per wave for all its idle
-// lanes, so a wave never idles behind its slowest ray.  Every ray writes one
-// result (a hit byte, or t for thickness) to a per-launch buffer; k_reduce
-// then folds each texel's S results in sample order (step2 + tangentspace),
-// the same split as the reference's step1/step2 dispatches.
-// ---------------------------------------------------------------------------
-enum { kAO = 0, kBN = 1, kThick = 2 };
-
-constexpr uint32_t kRefill = 16;           // refill a wave once this many lanes are idle
-constexpr float kSlabRel = 1.0f / 2097152.0f;  // 2^-21 = 8 units in the last place
-constexpr float kSlabAbs = 1e-37f;
-constexpr float kSlowDir = 9.094947e-13f;  // 2^-40: smaller direction components use the exact test
-
-struct Ray {
-    V3 o, d, inv;
-    bool slow;
-};
-
-// The reference's node test is RayAABB(o, d, box) < T with six IEEE
-// divisions (Shaders/ao_step1.comp:43-52, :176).  Here the slab distances are
-// first taken as (m - o) * RN(1/d), which lies within 3 units in the last
-// place of RN((m - o)/d) (two roundings of a correctly rounded reciprocal and
-// product against one of the quotient); max/min keep that relative bound.
-// With an 8-ulp margin the outcome is decided whenever it cannot flip, and
-// only the remaining boxes are retested with the reference's divisions, so
-// the decision is always the reference's.
-__device__ inline bool node_visit(const Ray& r, float4 lo, float4 hi, float T)
-{
-    if (!r.slow) {
-        const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
-        const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
-        const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
-        const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
-        const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
-        const float ea = __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
-        const float eb = __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
-        const float al = a - ea, ah = a + ea, bl = b - eb, bh = b + eb;
-        if (bl >= 0.0f && ah <= bl && ah < T) return true;
-        if (bh < 0.0f || al > bh || al >= T) return false;
-    }
-    return ray_aabb(r.o, r.d, lo, hi) < T;
-}
-
-// Conservative form of the same test for the culling tree: false only when
-// the box is missed for certain (its boxes are padded, so certainty suffices).
-__device__ inline bool node_maybe(const Ray& r, float4 lo, float4 hi, float T)
-{
-    const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
-    const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
-    const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
-    const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
-    const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
-    const float al = a - __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
-    const float bh = b + __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
-    return !(bh < 0.0f || al > bh || al >= T);
-}
-
-// AO and BN decide a ray on its first qualifying hit (ao_step1.comp:158,
-// bentnormals_step1.comp:170), and the reference finds that hit iff the
-// triangle's leaf is reachable: every node from the root down to it has slab
-// entry < maxD (before the first hit the running minimum is FLT_MAX, and once
-// a hit below maxD exists the ray is decided).  So a hit found through the
-// culling tree counts exactly when its reference leaf's chain passes the
-// reference test.
-__device__ inline bool chain_ok(const Ray& r, const DevScene& sc, uint32_t leaf, float T)
-{
-    for (uint32_t x = leaf; x != kMiss; x = sc.parent[x])
-        if (!node_visit(r, sc.nodes[2 * x], sc.nodes[2 * x + 1], T)) return false;
-    return true;
-}
-
-// Shaders/ao_step1.comp:109-147 with the per-triangle terms read from `tri`.
-// Returns t, or FLT_MAX.  Triangles whose t falls outside [tmin, tmax) cannot
-// change the caller's result and are rejected before the barycentrics.
-__device__ inline float hemi_tri_q(V3 o, V3 d, float4 q0, uint32_t k, const DevScene& sc, float tmin,
-                                   float tmax)
-{
-    const V3 n = xyz(q0);
-    const float nd = dot(d, n);
-    if (!(fabsf(nd) > 0)) return FLT_MAX;
-    const float pn = dot(o, n);
-    const float t = (q0.w - pn) / nd;
-    if (!(t >= 0 && t >= tmin && t < tmax)) return FLT_MAX;
-    const float4 q1 = sc.trest[3 * k], q2 = sc.trest[3 * k + 1], q3 = sc.trest[3 * k + 2];
-    const V3 p = o + d * t;
-    const V3 v2 = p - xyz(q1), v0 = xyz(q2), v1 = xyz(q3);
-    const float d00 = q1.w, d01 = q2.w, d11 = q3.w;
-    const float d20 = dot(v2, v0), d21 = dot(v2, v1);
-    const float denom = d00 * d11 - d01 * d01;
-    const float y = (d11 * d20 - d01 * d21) / denom;
-    const float z = (d00 * d21 - d01 * d20) / denom;
-    const float x = 1.0f - y - z;
-    return (x >= 0 && y >= 0 && y <= 1 && z >= 0 && z <= 1) ? t : FLT_MAX;
-}
-
-__device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmin, float tmax)
-{
-    return hemi_tri_q(o, d, sc.tri[k], k, sc, tmin, tmax);
-}
-
-struct Basis {
-    V3 o, d, tx, ty;
-};
-
-// Shaders/ao_step0.comp:39-56 for a texel whose mapping hit soup vertex t.
-__device__ inline Basis ray_basis(const DevScene& sc, float4 c, uint32_t t)
-{
-    Basis B;
-    const V3 p0 = xyz(sc.spos[t]), p1 = xyz(sc.spos[t + 1]), p2 = xyz(sc.spos[t + 2]);
-    B.o = p0 * c.y + p1 * c.z + p2 * c.w;
-    const V3 n0 = xyz(sc.snrm[t]), n1 = xyz(sc.snrm[t + 1]), n2 = xyz(sc.snrm[t + 2]);
-    B.d = normalize(n0 * c.y + n1 * c.z + n2 * c.w);
-    const V3 d = B.d;
-    B.ty = normalize(fabsf(d.x) > fabsf(d.y) ? mk(d.z, 0, -d.x) : mk(0, d.z, -d.y));
-    B.tx = cross(d, B.ty);
-    return B;
-}
-
-// Sample direction of ao_step1.comp:153-155 (thick_step1.comp:144-150 flips d).
-__device__ inline V3 sample_dir(const Basis& B, V3 d, float4 rs)
-{
-    return normalize(B.tx * rs.x + B.ty * rs.y + d * rs.z);
-}
-
-struct RayArgs {
-    DevScene sc;
-    const float4* __restrict__ coords;
-    const uint32_t* __restrict__ tidx;
-    const float4* __restrict__ samples;
-    uint8_t* __restrict__ hit;          // AO / BN: 1 when the ray hit (per ray of the launch)
-    float* __restrict__ tval;           // thickness: closest t, FLT_MAX when nothing was hit
-    unsigned int* __restrict__ counter;  // next ray of the launch to hand out
-    unsigned long long* __restrict__ stats;  // node tests, triangle tests, rays (COUNT only)
-    uint32_t S, P, t0, ntex;            // texels [t0, t0 + ntex), S rays each
-    float mind, maxd;
-};
-
-// One step of raycastBVH (Shaders/ao_step1.comp:166-193) in the reference's
-// stackless order.  AO: any hit with t in [minD, maxD) decides the ray
-// (ao_step1.comp:158); nodes are visited iff entry < maxD (the running minimum
-// only prunes nodes the maxD test already prunes).  BN: any hit with
-// t >= minD decides it (bentnormals_step1.comp:170).  Thickness: closest hit
-// with t >= minD over nodes with entry < min(best, maxD).
-template <int KIND, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
-{
-    const uint32_t total = A.ntex * A.S;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t n_nodes = A.sc.n_nodes;
-    Ray ray;
-    ray.o = ray.d = ray.inv = mk(0, 0, 0);
-    ray.slow = false;
-    uint32_t rid = 0, node = 0;
-    float best = FLT_MAX;
-    bool active = false, exhausted = false, ref_path = false;
-    uint32_t cn = 0, ct = 0, nr = 0;
-    for (;;) {
-        const uint64_t idle = __ballot(!active);
-        const uint32_t nidle = __popcll(idle);
-        if (!exhausted && (nidle >= kRefill || nidle == 64)) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(A.counter, nidle);
-            base = __shfl(base, 0);
-            if (base + nidle >= total) exhausted = true;
-            if (!active) {
-                const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
-                if (r < total) {
-                    const uint32_t q = r / A.S, s = r - q * A.S, g = A.t0 + q;
-                    const uint32_t t = A.tidx[g];
-                    if (t == kMiss) {
-                        if (KIND == kThick) A.tval[r] = FLT_MAX;
-                        else A.hit[r] = 0;
-                    } else {
-                        const Basis B = ray_basis(A.sc, A.coords[g], t);
-                        const V3 d = KIND == kThick ? -B.d : B.d;
-                        ray.o = B.o;
-                        ray.d = sample_dir(B, d, A.samples[(size_t)(g % A.P) * A.S + s]);
-                        ray.inv = mk(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
-                        ray.slow = !(fabsf(ray.d.x) >= kSlowDir && fabsf(ray.d.y) >= kSlowDir &&
-                                     fabsf(ray.d.z) >= kSlowDir);
-                        ref_path = ray.slow;  // tiny direction components: reference tree, exact test
-                        rid = r;
-                        node = 0;
-                        best = FLT_MAX;
-                        active = true;
-                        if (COUNT) ++nr;
-                    }
-                }
-            }
-        }
-        if (exhausted && __ballot(active) == 0) break;
-        if (active && !ref_path) {
-            // The culling tree.  AO / BN: a qualifying hit decides the ray iff
-            // its reference leaf is reachable (chain_ok with maxD).
-            // Thickness: a hit at t < best is taken when its leaf's chain
-            // passes with min(maxD, t): then the reference, whose running
-            // minimum never drops below the final one, visits that leaf too,
-            // and the minimum over reachable leaves is the reference's value.
-            // A leaf reachable with maxD but not with min(maxD, t) (a box
-            // entered no earlier than the hit) sends the ray to the reference
-            // tree instead.
-            const float4 lo = A.sc.cnodes[2 * node], hi = A.sc.cnodes[2 * node + 1];
-            const uint32_t w = __float_as_uint(hi.w);
-            const float T = KIND == kThick ? fminf(best, A.maxd) : A.maxd;
-            bool found = false, fallback = false;
-            if (COUNT) ++cn;
-            if (node_maybe(ray, lo, hi, T)) {
-                if (w & kLeafFlag) {
-                    const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
-                    const float tri_hi = KIND == kAO ? A.maxd : INFINITY;
-                    bool stop = false;
-                    for (uint32_t k0 = first; k0 < end && !stop; k0 += 4) {
-                        // the leaf's (n, a.n) records are contiguous: fetch four at once
-                        float4 q[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) q[j] = k0 + j < end ? A.sc.tri[k0 + j] : make_float4(0, 0, 0, 0);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const uint32_t k = k0 + j;
-                            if (stop || k >= end) continue;
-                            if (COUNT) ++ct;
-                            const float t = hemi_tri_q(ray.o, ray.d, q[j], k, A.sc, A.mind,
-                                                       KIND == kThick ? best : tri_hi);
-                            if (t == FLT_MAX) continue;
-                            const uint32_t leaf = A.sc.cref[node];
-                            if (KIND != kThick) {
-                                found = chain_ok(ray, A.sc, leaf, A.maxd);
-                                stop = true;  // the other triangles of the leaf share its chain
-                            } else if (chain_ok(ray, A.sc, leaf, fminf(A.maxd, t))) {
-                                best = t;
-                            } else {
-                                fallback = chain_ok(ray, A.sc, leaf, A.maxd);
-                                stop = true;  // unreachable leaf: none of its triangles count
-                            }
-                        }
-                    }
-                }
-                ++node;
-            } else {
-                node = (w & kLeafFlag) ? node + 1 : w;
-            }
-            if (fallback) {
-                ref_path = true;  // redo this ray on the reference tree
-                node = 0;
-                best = FLT_MAX;
-            } else if (found || node >= A.sc.n_cnodes) {
-                if (KIND == kThick) A.tval[rid] = best;
-                else A.hit[rid] = found ? 1 : 0;
-                active = false;
-            }
-        } else if (active) {
-            const float4 lo = A.sc.nodes[2 * node], hi = A.sc.nodes[2 * node + 1];
-            const uint32_t w = __float_as_uint(hi.w);
-            const float T = KIND == kThick ? fminf(best, A.maxd) : A.maxd;
-            bool done = false, found = false;
-            if (COUNT) ++cn;
-            if (node_visit(ray, lo, hi, T)) {
-                if (w & kLeafFlag) {
-                    const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
-                    const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : best);
-                    for (uint32_t k = first; k < end; ++k) {
-                        if (COUNT) ++ct;
-                        const float t = hemi_tri(ray.o, ray.d, k, A.sc, A.mind, KIND == kThick ? best : tri_hi);
-                        if (t != FLT_MAX) {
-                            if (KIND != kThick) { found = true; break; }
-                            best = t;
-                        }
-                    }
-                }
-                ++node;
-            } else {
-                node = (w & kLeafFlag) ? node + 1 : w;
-            }
-            if (found || node >= n_nodes) done = true;
-            if (done) {
-                if (KIND == kThick) A.tval[rid] = best;
-                else A.hit[rid] = found ? 1 : 0;
-                active = false;
-            }
-        }
-    }
-    if (COUNT) {
-        unsigned long long a = cn, b = ct, c = nr;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_down(a, off);
-            b += __shfl_down(b, off);
-            c += __shfl_down(c, off);
-        }
-        if (lane == 0) {
-            atomicAdd(&A.stats[0], a);
-            atomicAdd(&A.stats[1], b);
-            atomicAdd(&A.stats[2], c);
-        }
-    }
-}
-
-struct ReduceArgs {
-    DevScene sc;
-    const float4* __restrict__ coords;
-    const uint32_t* __restrict__ tidx;
-    const float4* __restrict__ samples;
-    const float4* __restrict__ pixt;    // n, t, b per texel (BN tangent space) or null
-    const uint8_t* __restrict__ hit;
-    const float* __restrict__ tval;
-    float* __restrict__ out;            // 1 or 3 floats per texel (global texel index)
-    uint32_t* __restrict__ counts;      // per texel
-    uint32_t S, P, t0, ntex;
-    float maxd;
-    int tangent;
-};
-
-// step2 reductions in sample order (ao_step2.comp:24-29, thick_step2.comp:24-29,
-// bentnormals_step2.comp:25-34) and tangentspace.comp:22-41.  Texels whose
-// mapping missed get AO 1, thickness maxD and bent normal (0,0,0).
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
-{
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= A.ntex) return;
-    const uint32_t g = A.t0 + q, S = A.S;
-    const size_t base = (size_t)q * S;
-    const uint32_t t = A.tidx[g];
-    uint32_t cnt = 0;
-    if (KIND == kAO) {
-        if (t != kMiss)
-            for (uint32_t s = 0; s < S; ++s) cnt += A.hit[base + s];
-        A.out[g] = t == kMiss ? 1.0f : 1.0f - (float)cnt / (float)S;
-    } else if (KIND == kThick) {
-        float acc = 0;
-        if (t != kMiss)
-            for (uint32_t s = 0; s < S; ++s) {
-                const float v = A.tval[base + s];
-                if (v != FLT_MAX) ++cnt;
-                acc += v != FLT_MAX ? v : A.maxd;
-            }
-        A.out[g] = t == kMiss ? A.maxd : acc / (float)S;
-    } else {
-        V3 r = mk(0, 0, 0);
-        if (t != kMiss) {
-            const Basis B = ray_basis(A.sc, A.coords[g], t);
-            const float4* smp = A.samples + (size_t)(g % A.P) * S;
-            V3 acc = mk(0, 0, 0);
-            for (uint32_t s = 0; s < S; ++s) {
-                if (A.hit[base + s]) { ++cnt; continue; }   // adds vec3(0): acc never holds -0
-                acc = acc + sample_dir(B, B.d, smp[s]);
-            }
-            r = normalize(acc);
-            if (A.tangent) {
-                const V3 n = xyz(A.pixt[3 * g]), tt = xyz(A.pixt[3 * g + 1]), b = xyz(A.pixt[3 * g + 2]);
-                const V3 d0 = mk(n.z * b.y - n.y * b.z, n.x * b.z - n.z * b.x, n.y * b.x - n.x * b.y);
-                const V3 d1 = mk(tt.z * n.y - tt.y * n.z, tt.x * n.z - n.x * tt.z, n.x * tt.y - tt.x * n.y);
-                const V3 d2 = mk(tt.y * b.z - tt.z * b.y, tt.z * b.x - tt.x * b.z, tt.x * b.y - tt.y * b.x);
-                r = normalize(mk(dot(r, d0), dot(r, d1), dot(r, d2)));
-            }
-        }
-        A.out[3 * g] = r.x;
-        A.out[3 * g + 1] = r.y;
-        A.out[3 * g + 2] = r.z;
-    }
-    A.counts[g] = cnt;
-}
-
-// Clears n 32-bit words (a kernel rather than hipMemsetAsync, so the whole
-// pass can be captured into a HIP graph).
-__global__ void k_zero(uint32_t* p, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-// ---------------------------------------------------------------------------
-// Texel passes: heights.comp, positions.comp, normals.comp (+ tangentspace).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_height(const float4* __restrict__ coords, uint32_t begin,
-                                                    uint32_t end, float* __restrict__ out)
-{
-    const uint32_t g = begin + blockIdx.x * kBlock + threadIdx.x;
-    if (g >= end) return;
-    const float h = coords[g].x;
-    out[g] = h != FLT_MAX ? h : 0.0f;
-}
-
-template <bool NORMALS>
-__global__ __launch_bounds__(kBlock) void k_attr(const float4* __restrict__ coords,
-                                                  const uint32_t* __restrict__ tidx,
-                                                  const float4* __restrict__ soup,
-                                                  const float4* __restrict__ pixt, uint32_t begin,
-                                                  uint32_t end, int tangent, float* __restrict__ out)
-{
-    const uint32_t g = begin + blockIdx.x * kBlock + threadIdx.x;
-    if (g >= end) return;
-    const uint32_t t = tidx[g];
-    V3 r = mk(0, 0, 0);
-    if (t != kMiss) {
-        const float4 c = coords[g];
-        r = xyz(soup[t]) * c.y + xyz(soup[t + 1]) * c.z + xyz(soup[t + 2]) * c.w;
-        if (NORMALS) {
-            r = normalize(r);
-            if (tangent) {
-                const V3 n = xyz(pixt[3 * g]), tt = xyz(pixt[3 * g + 1]), b = xyz(pixt[3 * g + 2]);
-                const V3 d0 = mk(n.z * b.y - n.y * b.z, n.x * b.z - n.z * b.x, n.y * b.x - n.x * b.y);
-                const V3 d1 = mk(tt.z * n.y - tt.y * n.z, tt.x * n.z - n.x * tt.z, n.x * tt.y - tt.x * n.y);
-                const V3 d2 = mk(tt.y * b.z - tt.z * b.y, tt.z * b.x - tt.x * b.z, tt.x * b.y - tt.y * b.x);
-                r = normalize(mk(dot(r, d0), dot(r, d1), dot(r, d2)));
-            }
-        }
-    }
-    out[3 * g] = r.x;
-    out[3 * g + 1] = r.y;
-    out[3 * g + 2] = r.z;
-}
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -656,13 +110,15 @@ struct fornos_scene {
     DevScene d{};
     float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr, *cnodes = nullptr;
     uint32_t *parent = nullptr, *cref = nullptr;
+    float4* wide = nullptr;
+    uint4* wleaf = nullptr;
     FlatBVH bvh;
     fornos_scene_info info{};
     ~fornos_scene()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
-        dev_free(cnodes); dev_free(parent); dev_free(cref);
+        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -728,12 +184,15 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.coords = mm->coords;
     A.tidx = mm->tidx;
     A.samples = s->samples;
-    A.hit = KIND == kThick ? nullptr : (uint8_t*)s->raybuf;
+    const uint32_t mwords = (S + 31) / 32;
+    A.cnt = s->counts;
+    A.mask = KIND == kBN ? (uint32_t*)s->raybuf : nullptr;
     A.tval = KIND == kThick ? (float*)s->raybuf : nullptr;
     A.counter = s->counter;
     A.stats = s->stats;
     A.S = S;
     A.P = s->P;
+    A.mwords = mwords;
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
     ReduceArgs R;
@@ -742,21 +201,28 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     R.tidx = mm->tidx;
     R.samples = s->samples;
     R.pixt = mm->pixt;
-    R.hit = A.hit;
+    R.mask = A.mask;
     R.tval = A.tval;
     R.out = s->out;
     R.counts = s->counts;
     R.S = S;
     R.P = s->P;
+    R.mwords = mwords;
     R.maxd = s->p.max_distance;
     R.tangent = s->p.tangent_space;
-    if (count) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc->stream, (uint32_t*)s->stats, 6u);
+    auto fill = [&](void* p, size_t n, uint32_t v) {
+        hipLaunchKernelGGL(k_fill, dim3(div_up(n, kBlock)), dim3(kBlock), 0, sc->stream, (uint32_t*)p, n, v);
+    };
+    if (count) fill(s->stats, 6, 0);
     for (uint32_t c0 = t0; c0 < t1; c0 += s->chunk_texels) {
         const uint32_t nt = std::min(s->chunk_texels, t1 - c0);
         A.t0 = R.t0 = c0;
         A.ntex = R.ntex = nt;
         const uint32_t blocks = std::min<uint32_t>(sc->persistent_blocks, div_up((uint64_t)nt * S, kBlock));
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc->stream, s->counter, 1u);
+        fill(s->counter, 1, 0);
+        if (KIND == kAO) fill(s->counts + c0, nt, 0);
+        if (KIND == kBN) fill(s->raybuf, (size_t)nt * mwords, 0);
+        if (KIND == kThick) fill(s->raybuf, (size_t)nt * S, 0x7f7fffffu);  // FLT_MAX
         hipEvent_t* ev = nullptr;
         if (s->timed) {
             s->events.resize(s->events.size() + 3, nullptr);
@@ -890,6 +356,12 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     sc->d.cref = sc->cref;
     sc->d.parent = sc->parent;
     sc->d.n_cnodes = cull.n_nodes;
+    if (fornos_status st = dev_alloc(&sc->wide, (size_t)8 * cull.n_wide)) return st;
+    if (fornos_status st = dev_alloc(&sc->wleaf, cull.wide_leaves.size() / 4)) return st;
+    HIP_TRY(hipMemcpy(sc->wide, cull.wide.data(), sizeof(uint32_t) * cull.wide.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->wleaf, cull.wide_leaves.data(), sizeof(uint32_t) * cull.wide_leaves.size(), hipMemcpyHostToDevice));
+    sc->d.wide = sc->wide;
+    sc->d.wleaf = sc->wleaf;
     if (fornos_status st = dev_alloc(&sc->nodes, (size_t)2 * B.n_nodes)) return st;
     if (fornos_status st = dev_alloc(&sc->tri, tri.size())) return st;
     if (fornos_status st = dev_alloc(&sc->spos, spos.size())) return st;
@@ -1074,14 +546,16 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         if (fornos_status st = dev_alloc(&s->counts, mm->n)) return st;
         if (fornos_status st = dev_alloc(&s->stats, 3)) return st;
         if (fornos_status st = dev_alloc(&s->counter, 1)) return st;
-        // per-ray results of one k_rays launch: at most 2^28 rays (256 MiB of
-        // hit bytes, 1 GiB of thickness t values)
+        // per-ray results of one k_rays launch of at most 2^28 rays: hit bits
+        // (BN, 32 MiB) or t values (thickness, 1 GiB); AO counts in place
         s->chunk_texels = std::max<uint32_t>(1, (1u << 28) / S);
-        const size_t rays = (size_t)std::min<uint32_t>(s->chunk_texels, std::max<uint32_t>(1, mm->n)) * S;
-        const size_t elem = kind == FORNOS_SOLVER_THICKNESS ? sizeof(float) : 1;
-        uint8_t* rb = nullptr;
-        if (fornos_status st = dev_alloc(&rb, rays * elem)) return st;
-        s->raybuf = rb;
+        const size_t tex = std::min<uint32_t>(s->chunk_texels, std::max<uint32_t>(1, mm->n));
+        if (kind != FORNOS_SOLVER_AO) {
+            const size_t words = kind == FORNOS_SOLVER_THICKNESS ? tex * S : tex * ((S + 31) / 32);
+            uint32_t* rb = nullptr;
+            if (fornos_status st = dev_alloc(&rb, words)) return st;
+            s->raybuf = rb;
+        }
     } else {
         s->work_total = mm->work_count;
     }
