@@ -84,6 +84,9 @@ _SIGS = [
     ("fornos_scene_export_soup", None, [H, P, P]),
     ("fornos_scene_destroy", "void", [H]),
     ("fornos_meshmap_create", None, [H, C.POINTER(TexelView), C.c_int, C.POINTER(H)]),
+    ("fornos_meshmap_create_from_mesh", None, [H, C.POINTER(MeshView), C.POINTER(MeshView), C.c_uint32,
+                                                C.c_uint32, C.c_int, C.POINTER(H)]),
+    ("fornos_meshmap_read_texels", None, [H, P, P, P, P, P, P]),
     ("fornos_meshmap_run_step", None, [H, C.POINTER(C.c_int)]),
     ("fornos_meshmap_run_all", None, [H, C.POINTER(C.c_float)]),
     ("fornos_meshmap_progress", None, [H, C.POINTER(C.c_float)]),

@@ -184,6 +184,26 @@ class MeshMapping(_Handle):
         check(lib().fornos_meshmap_create(scene.handle, C.byref(tv), int(bool(cullBackfaces)), C.byref(self._h)))
         return self
 
+    def init_from_mesh(self, lo, dirs, width, height, scene, cullBackfaces=False):
+        """MeshMapping::init with the texel set rasterised on the device."""
+        self.close()
+        self.texels, self.scene = None, scene
+        lv = lo.mesh_view()
+        dv = dirs.mesh_view() if dirs is not None else None
+        check(lib().fornos_meshmap_create_from_mesh(scene.handle, C.byref(lv), C.byref(dv) if dv else None,
+                                                    int(width), int(height), int(bool(cullBackfaces)),
+                                                    C.byref(self._h)))
+        return self
+
+    def read_texels(self):
+        n = self.count
+        outs = [np.empty((n, 3), np.float32) for _ in range(5)]
+        idx = np.empty(n, np.uint32)
+        check(lib().fornos_meshmap_read_texels(self._h, *[ptr(o) for o in outs], ptr(idx)))
+        d = dict(zip(["positions", "directions", "normals", "tangents", "bitangents"], outs))
+        d["indices"] = idx
+        return d
+
     def runStep(self):
         done = C.c_int()
         check(lib().fornos_meshmap_run_step(self._h, C.byref(done)))

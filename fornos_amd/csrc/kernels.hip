@@ -21,6 +21,9 @@
 #include "hemi.h"
 #include "mapping.h"
 #include "texel.h"
+#include "raster.h"
+
+#include <hipcub/hipcub.hpp>
 
 #pragma clang fp contract(off)
 
@@ -130,11 +133,11 @@ struct fornos_meshmap {
     uint32_t offset = 0;      // texels mapped so far
     int cull = 1;
     float4 *pix = nullptr, *pixt = nullptr, *coords = nullptr;
-    uint32_t* tidx = nullptr;
+    uint32_t *tidx = nullptr, *indices = nullptr;
     ~fornos_meshmap()
     {
         if (sc && sc->stream) (void)hipStreamSynchronize(sc->stream);
-        dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx);
+        dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx); dev_free(indices);
     }
 };
 
@@ -447,9 +450,137 @@ fornos_status fornos_meshmap_create(fornos_scene* sc, const fornos_texel_view* t
         if (fornos_status st = dev_alloc(&mm->pixt, pt.size())) return st;
         HIP_TRY(hipMemcpy(mm->pixt, pt.data(), sizeof(float4) * pt.size(), hipMemcpyHostToDevice));
     }
+    if (tv->indices) {
+        if (fornos_status st = dev_alloc(&mm->indices, mm->n)) return st;
+        HIP_TRY(hipMemcpy(mm->indices, tv->indices, sizeof(uint32_t) * mm->n, hipMemcpyHostToDevice));
+    }
     if (fornos_status st = dev_alloc(&mm->coords, mm->n)) return st;
     if (fornos_status st = dev_alloc(&mm->tidx, mm->n)) return st;
     *out = mm.release();
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_create_from_mesh(fornos_scene* sc, const fornos_mesh_view* lo,
+                                              const fornos_mesh_view* dirs, uint32_t W, uint32_t H, int cull,
+                                              fornos_meshmap** out)
+{
+    if (!sc || !lo || !out) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: null argument");
+    *out = nullptr;
+    if (W == 0 || H == 0 || W > 16384 || H > 16384)
+        return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: map size must be in [1, 16384]");
+    if (dirs && dirs->n_vertices != lo->n_vertices)
+        return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: mapping mesh must be a copy of the low-poly mesh");
+    // corner records of rasterTriangle (Src/compute.cpp:85-126)
+    const uint32_t nt = lo->n_triangles;
+    std::vector<float> tris((size_t)kTriFloats * nt, 0.0f);
+    for (uint32_t t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = lo->triangles[3 * t + k];
+            if (v >= lo->n_vertices) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: bad vertex index");
+            const uint32_t pi = lo->vertices[3 * v], ti = lo->vertices[3 * v + 1], ni = lo->vertices[3 * v + 2];
+            if (ti == UINT32_MAX || ni == UINT32_MAX || !lo->texcoords || !lo->normals)
+                return set_error(FORNOS_INVALID_ARG, "Low poly mesh is missing texture coordinates or normals information");
+            if (pi >= lo->n_positions || ti >= lo->n_texcoords || ni >= lo->n_normals)
+                return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: vertex index out of range");
+            float* c = &tris[(size_t)t * kTriFloats + k * kCornerFloats];
+            std::memcpy(c, &lo->positions[3 * (size_t)pi], 12);
+            std::memcpy(c + 3, &lo->texcoords[2 * (size_t)ti], 8);
+            std::memcpy(c + 5, &lo->normals[3 * (size_t)ni], 12);
+            const float* d = &lo->normals[3 * (size_t)ni];
+            if (dirs) {
+                const uint32_t di = dirs->vertices[3 * v + 2];
+                if (di >= dirs->n_normals) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: bad mapping normal index");
+                d = &dirs->normals[3 * (size_t)di];
+            }
+            std::memcpy(c + 8, d, 12);
+            if (lo->tangents && lo->bitangents) {
+                std::memcpy(c + 11, &lo->tangents[3 * (size_t)v], 12);
+                std::memcpy(c + 14, &lo->bitangents[3 * (size_t)v], 12);
+            }
+        }
+    }
+    std::unique_ptr<fornos_meshmap> mm(new fornos_meshmap());
+    mm->sc = sc;
+    mm->width = W;
+    mm->height = H;
+    mm->cull = cull ? 1 : 0;
+    hipStream_t st = sc->stream;
+    const uint64_t npx = (uint64_t)W * H;
+    struct Tmp {
+        float* tris = nullptr;
+        float* map = nullptr;
+        uint32_t *owner = nullptr, *flags = nullptr, *slot = nullptr;
+        void* scan = nullptr;
+        ~Tmp() { dev_free(tris); dev_free(map); dev_free(owner); dev_free(flags); dev_free(slot); dev_free(scan); }
+    } tmp;
+    if (fornos_status s = dev_alloc(&tmp.tris, tris.size())) return s;
+    if (fornos_status s = dev_alloc(&tmp.map, 15 * npx)) return s;
+    if (fornos_status s = dev_alloc(&tmp.owner, npx)) return s;
+    if (fornos_status s = dev_alloc(&tmp.flags, npx)) return s;
+    if (fornos_status s = dev_alloc(&tmp.slot, npx)) return s;
+    HIP_TRY(hipMemcpyAsync(tmp.tris, tris.data(), sizeof(float) * tris.size(), hipMemcpyHostToDevice, st));
+    const uint32_t pblocks = div_up(npx, kBlock);
+    hipLaunchKernelGGL(k_fill, dim3(pblocks), dim3(kBlock), 0, st, tmp.owner, (size_t)npx, 0u);
+    hipLaunchKernelGGL(k_fill, dim3(div_up(15 * npx, kBlock)), dim3(kBlock), 0, st, (uint32_t*)tmp.map, (size_t)(15 * npx), 0u);
+    RasterArgs R{tmp.tris, nt, W, H, tmp.owner, tmp.map};
+    if (nt) hipLaunchKernelGGL(k_raster_bid, dim3(nt), dim3(kBlock), 0, st, R);
+    hipLaunchKernelGGL(k_raster_write, dim3(pblocks), dim3(kBlock), 0, st, R);
+    hipLaunchKernelGGL(k_texel_flags, dim3(pblocks), dim3(kBlock), 0, st, tmp.map + 3 * npx, npx, tmp.flags);
+    size_t scan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, tmp.flags, tmp.slot, (int)npx, st));
+    uint8_t* sb = nullptr;
+    if (fornos_status s = dev_alloc(&sb, scan_bytes)) return s;
+    tmp.scan = sb;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.scan, scan_bytes, tmp.flags, tmp.slot, (int)npx, st));
+    uint32_t last_slot = 0, last_flag = 0;
+    HIP_TRY(hipMemcpyAsync(&last_slot, tmp.slot + npx - 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&last_flag, tmp.flags + npx - 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    mm->n = last_slot + last_flag;
+    mm->work_count = div_up(mm->n, kGroup) * kGroup;
+    if (fornos_status s = dev_alloc(&mm->pix, (size_t)2 * mm->n)) return s;
+    if (fornos_status s = dev_alloc(&mm->pixt, (size_t)3 * mm->n)) return s;
+    if (fornos_status s = dev_alloc(&mm->indices, mm->n)) return s;
+    if (fornos_status s = dev_alloc(&mm->coords, mm->n)) return s;
+    if (fornos_status s = dev_alloc(&mm->tidx, mm->n)) return s;
+    hipLaunchKernelGGL(k_texel_compact, dim3(pblocks), dim3(kBlock), 0, st, tmp.map, npx, tmp.flags, tmp.slot,
+                       mm->pix, mm->pixt, mm->indices);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = mm.release();
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_read_texels(const fornos_meshmap* mm, float* p, float* d, float* n, float* t, float* b,
+                                         uint32_t* indices)
+{
+    if (!mm) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_read_texels: null handle");
+    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
+    if (mm->n == 0) return FORNOS_OK;
+    std::vector<float4> buf((size_t)3 * mm->n);
+    auto unpack = [&](float* dst, size_t stride, size_t off) {
+        if (!dst) return;
+        for (size_t i = 0; i < mm->n; ++i) {
+            const float4 v = buf[stride * i + off];
+            dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z;
+        }
+    };
+    if (p || d) {
+        HIP_TRY(hipMemcpy(buf.data(), mm->pix, sizeof(float4) * 2 * mm->n, hipMemcpyDeviceToHost));
+        unpack(p, 2, 0);
+        unpack(d, 2, 1);
+    }
+    if (n || t || b) {
+        if (!mm->pixt) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_read_texels: the texel set has no tangent frames");
+        HIP_TRY(hipMemcpy(buf.data(), mm->pixt, sizeof(float4) * 3 * mm->n, hipMemcpyDeviceToHost));
+        unpack(n, 3, 0);
+        unpack(t, 3, 1);
+        unpack(b, 3, 2);
+    }
+    if (indices) {
+        if (!mm->indices) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_read_texels: the texel set has no MapUV indices");
+        HIP_TRY(hipMemcpy(indices, mm->indices, sizeof(uint32_t) * mm->n, hipMemcpyDeviceToHost));
+    }
     return FORNOS_OK;
 }
 

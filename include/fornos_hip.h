@@ -120,6 +120,22 @@ typedef struct fornos_meshmap fornos_meshmap;
 /* MeshMapping::init(map, mesh, bvh, cullBackfaces) (Src/meshmapping.cpp:128-184). */
 fornos_status fornos_meshmap_create(fornos_scene *scene, const fornos_texel_view *texels,
                                     int cull_backfaces, fornos_meshmap **out);
+/* MeshMapping::init with the texel set built on the device: MapUV::fromMeshes
+ * (Src/compute.cpp:358, :75-193) + CompressedMapUV (:372-416) + the uploads
+ * of MeshMapping::init (Src/meshmapping.cpp:133-147), in one call.  lo: the
+ * low-poly mesh (texcoords and normals required; per-vertex tangents and
+ * bitangents optional); dirs: the mapping copy whose normals give the ray
+ * directions (the Smooth method), or NULL to use lo's normals (fromMesh).
+ * The texel set equals the host one (fornos_texels_create) bit for bit. */
+fornos_status fornos_meshmap_create_from_mesh(fornos_scene *scene, const fornos_mesh_view *lo,
+                                              const fornos_mesh_view *dirs, uint32_t width,
+                                              uint32_t height, int cull_backfaces,
+                                              fornos_meshmap **out);
+/* Readback of the texel set held by a mesh mapping (3 floats per texel each,
+ * plus the MapUV index of each texel); any pointer may be NULL. */
+fornos_status fornos_meshmap_read_texels(const fornos_meshmap *mm, float *positions,
+                                         float *directions, float *normals, float *tangents,
+                                         float *bitangents, uint32_t *indices);
 /* MeshMapping::runStep (Src/meshmapping.cpp:186-218): one chunk of at most
  * 131072 texels; *done = 1 once every texel is mapped. */
 fornos_status fornos_meshmap_run_step(fornos_meshmap *mm, int *done);
