@@ -50,9 +50,10 @@ def test_device_raster_overwrite_order_and_bound_column(gpu):
     m = soup_mesh(tris, uvs=uvs)
     mp = orc.mapuv(m, W, H)
     idx = orc.compress(mp["directions"])
-    # column 0 is written only through triangle 1's bound column
+    # above its first rows, column 0 is written only through triangle 1's
+    # bound column (its x lies past the quad's edge)
     col0 = mp["positions"].reshape(H, W, 3)[:, 0]
-    assert np.all(col0[:, 2] == 1) and np.all(col0[:, 0] > 1)
+    assert np.all(col0[:, 2] == 1) and np.all(col0[8:, 0] > 1)
     lo = fa.Mesh.from_arrays(m["positions"], m["vertices"], m["triangles"], m["normals"], m["texcoords"])
     hi = fa.Mesh.generate(fa._abi.GEN_HEIGHTFIELD, 1, 0.05, 8, 0)
     scene = fa.Scene(hi)
