@@ -6,11 +6,11 @@ the seeded bake workloads.
 """
 from . import _abi
 from ._abi import FornosError, SYMBOLS, lib
-from .api import (AmbientOcclusionSolver, BentNormalsSolver, CompressedMapUV, HeightSolver, Mesh,
+from .api import (AmbientOcclusionSolver, Batch, BentNormalsSolver, CompressedMapUV, HeightSolver, Mesh,
                   MeshMapping, NormalsSolver, PositionSolver, Scene, ThicknessSolver,
                   device_count, sample_table)
 from .configs import CONFIGS, prepare
 
-__all__ = ["AmbientOcclusionSolver", "BentNormalsSolver", "CompressedMapUV", "HeightSolver", "Mesh",
+__all__ = ["AmbientOcclusionSolver", "Batch", "BentNormalsSolver", "CompressedMapUV", "HeightSolver", "Mesh",
            "MeshMapping", "NormalsSolver", "PositionSolver", "Scene", "ThicknessSolver",
            "device_count", "sample_table", "CONFIGS", "prepare", "FornosError", "SYMBOLS", "lib"]

@@ -347,5 +347,40 @@ class ThicknessSolver(Solver):
         super().__init__(sampleCount, minDistance, maxDistance, seed=seed, table=table)
 
 
+class Batch(_Handle):
+    """Several AO solvers run as one persistent launch over a device job table,
+    captured in a HIP graph (include/fornos_hip.h, fornos_batch_*).  No
+    reference counterpart: Src/fornos.cpp:239-251 runs one solver step per
+    frame.  Each solver's results equal its own run_all bit for bit and are
+    read with its getResults()."""
+    _destroy = "fornos_batch_destroy"
+
+    def __init__(self, solvers):
+        super().__init__()
+        self.solvers = list(solvers)  # the batch holds raw solver handles: keep them alive
+        arr = (C.c_void_p * len(self.solvers))(*[s.handle.value for s in self.solvers])
+        check(lib().fornos_batch_create(arr, len(self.solvers), C.byref(self._h)))
+
+    def set_launch(self, blocks_per_cu, refill):
+        check(lib().fornos_batch_set_launch(self._h, int(blocks_per_cu), int(refill)))
+
+    def get_launch(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        check(lib().fornos_batch_get_launch(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, fraction=0.05):
+        """Times every launch-shape candidate on a fraction of each job's texels,
+        keeps the fastest; returns (blocks_per_cu, refill, ms)."""
+        a, b, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().fornos_batch_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(ms)))
+        return a.value, b.value, ms.value
+
+    def run(self):
+        ms = C.c_float()
+        check(lib().fornos_batch_run(self._h, C.byref(ms)))
+        return ms.value
+
+
 def device_count():
     return lib().fornos_device_count()

@@ -31,7 +31,7 @@ namespace {
 
 enum { kAO = 0, kBN = 1, kThick = 2 };
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
-constexpr uint32_t kRefill = 16;  // refill a wave once this many lanes are idle
+constexpr uint32_t kRefill = 16;  // default: refill a wave once this many lanes are idle
 constexpr uint32_t kStack = 24;   // 4-wide traversal stack entries per lane
 
 struct RayArgs {
@@ -46,6 +46,7 @@ struct RayArgs {
     unsigned long long* __restrict__ stats;  // box tests, triangle tests, rays (COUNT only)
     uint32_t S, P, t0, ntex, mwords;    // texels [t0, t0 + ntex), S rays each
     float mind, maxd;
+    uint32_t refill;                    // refill a wave once this many lanes are idle
 };
 
 struct Lane {
@@ -209,12 +210,18 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
     if (L.node >= n_nodes) finish<KIND>(A, L, false);
 }
 
-template <int KIND, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
+// BATCH: `jobs` holds njobs independent passes (each its own scene, texels,
+// samples, counter and outputs).  A wave works on one job at a time -- the
+// job index is wave-uniform, so its parameters live in scalar registers --
+// and moves on once the job's counter is exhausted and its lanes are idle.
+template <int KIND, bool COUNT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __restrict__ jobs, uint32_t njobs)
 {
     __shared__ uint32_t stk_node[kStack * kBlock];
     __shared__ float stk_t[KIND == kThick ? kStack * kBlock : 1];
-    const uint32_t total = A.ntex * A.S;
+    uint32_t job = 0;
+    RayArgs A = BATCH ? jobs[0] : A0;
+    uint32_t total = A.ntex * A.S;
     const uint32_t lane = threadIdx.x & 63;
     Lane L;
     L.ray.o = L.ray.d = L.ray.inv = mk(0, 0, 0);
@@ -229,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
         // refill idle lanes, one atomic per wave
         const uint64_t idle = __ballot(!L.active);
         const uint32_t nidle = __popcll(idle);
-        if (!exhausted && (nidle >= kRefill || nidle == 64)) {
+        if (!exhausted && (nidle >= A.refill || nidle == 64)) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(A.counter, nidle);
             base = __shfl(base, 0);
@@ -260,7 +267,15 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A)
                 }
             }
         }
-        if (exhausted && __ballot(L.active) == 0) break;
+        if (exhausted && __ballot(L.active) == 0) {
+            if (!BATCH) break;
+            job = __builtin_amdgcn_readfirstlane(job + 1);
+            if (job >= njobs) break;
+            A = jobs[job];
+            total = A.ntex * A.S;
+            exhausted = false;
+            continue;
+        }
         // phase 1: walk the 4-wide tree until every lane holds a leaf or is done
         for (;;) {
             const bool walk = L.active && L.mode == kWideTree && (L.cur == kNoChild || !(L.cur & kLeafFlag));

@@ -198,6 +198,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.mwords = mwords;
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
+    A.refill = kRefill;
     ReduceArgs R;
     R.sc = sc->d;
     R.coords = mm->coords;
@@ -233,10 +234,11 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
             for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
             HIP_TRY(hipEventRecord(ev[0], sc->stream));
         }
+        const RayArgs* no_jobs = nullptr;
         if (count)
-            hipLaunchKernelGGL((k_rays<KIND, true>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
+            hipLaunchKernelGGL((k_rays<KIND, true, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A, no_jobs, 0u);
         else
-            hipLaunchKernelGGL((k_rays<KIND, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A);
+            hipLaunchKernelGGL((k_rays<KIND, false, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A, no_jobs, 0u);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
         if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
@@ -804,5 +806,199 @@ fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64
 }
 
 void fornos_solver_destroy(fornos_solver* s) { delete s; }
+
+}  // extern "C"
+
+struct fornos_batch {
+    std::vector<fornos_solver*> solvers;
+    std::vector<RayArgs> jobs;
+    hipStream_t stream = nullptr;
+    RayArgs* djobs = nullptr;
+    uint32_t* counters = nullptr;
+    int cus = 256;
+    uint32_t blocks_per_cu = 8, refill = kRefill;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    ~fornos_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        dev_free(djobs);
+        dev_free(counters);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Enqueues one batched pass: counters and AO counts cleared, one k_rays over
+// the job table (jobs[i].ntex texels each), one k_reduce per job.
+fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, RayArgs* djobs)
+{
+    hipStream_t st = b->stream;
+    const uint32_t nj = (uint32_t)jobs.size();
+    hipLaunchKernelGGL(k_fill, dim3(div_up(nj, kBlock)), dim3(kBlock), 0, st, b->counters, (size_t)nj, 0u);
+    for (const RayArgs& J : jobs)
+        if (J.ntex) hipLaunchKernelGGL(k_fill, dim3(div_up(J.ntex, kBlock)), dim3(kBlock), 0, st, J.cnt, (size_t)J.ntex, 0u);
+    uint64_t rays = 0;
+    for (const RayArgs& J : jobs) rays += (uint64_t)J.ntex * J.S;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)b->cus * b->blocks_per_cu, div_up(rays, kBlock));
+    if (blocks) hipLaunchKernelGGL((k_rays<kAO, false, true>), dim3(blocks), dim3(kBlock), 0, st, jobs[0], djobs, nj);
+    for (uint32_t j = 0; j < nj; ++j) {
+        const fornos_solver* s = b->solvers[j];
+        ReduceArgs R{};
+        R.sc = jobs[j].sc;
+        R.tidx = jobs[j].tidx;
+        R.out = s->out;
+        R.counts = s->counts;
+        R.S = jobs[j].S;
+        R.t0 = 0;
+        R.ntex = jobs[j].ntex;
+        if (R.ntex) hipLaunchKernelGGL(k_reduce<kAO>, dim3(div_up(R.ntex, kBlock)), dim3(kBlock), 0, st, R);
+    }
+    HIP_TRY(hipGetLastError());
+    return FORNOS_OK;
+}
+
+void batch_drop_graph(fornos_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, fornos_batch** out)
+{
+    if (!solvers || !out || n == 0) return set_error(FORNOS_INVALID_ARG, "fornos_batch_create: no solvers");
+    *out = nullptr;
+    std::unique_ptr<fornos_batch> b(new fornos_batch());
+    for (uint32_t j = 0; j < n; ++j) {
+        fornos_solver* s = solvers[j];
+        if (!s || s->kind != FORNOS_SOLVER_AO)
+            return set_error(FORNOS_UNSUPPORTED, "fornos_batch_create: every job must be an AO solver");
+        if ((uint64_t)s->mm->n * s->p.sample_count >= (1ull << 31))
+            return set_error(FORNOS_UNSUPPORTED, "fornos_batch_create: a job has 2^31 rays or more");
+        b->solvers.push_back(s);
+    }
+    HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->solvers[0]->mm->sc->device));
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    if (fornos_status st = dev_alloc(&b->counters, n)) return st;
+    if (fornos_status st = dev_alloc(&b->djobs, n)) return st;
+    for (uint32_t j = 0; j < n; ++j) {
+        const fornos_solver* s = b->solvers[j];
+        RayArgs A{};
+        A.sc = s->mm->sc->d;
+        A.coords = s->mm->coords;
+        A.tidx = s->mm->tidx;
+        A.samples = s->samples;
+        A.cnt = s->counts;
+        A.counter = b->counters + j;
+        A.stats = s->stats;
+        A.S = s->p.sample_count;
+        A.P = s->P;
+        A.t0 = 0;
+        A.ntex = s->mm->n;
+        A.mwords = (A.S + 31) / 32;
+        A.mind = s->p.min_distance;
+        A.maxd = s->p.max_distance;
+        A.refill = b->refill;
+        b->jobs.push_back(A);
+    }
+    HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * n, hipMemcpyHostToDevice));
+    *out = b.release();
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_set_launch(fornos_batch* b, uint32_t blocks_per_cu, uint32_t refill)
+{
+    if (!b || blocks_per_cu < 1 || blocks_per_cu > 16 || refill < 1 || refill > 64)
+        return set_error(FORNOS_INVALID_ARG, "fornos_batch_set_launch: blocks_per_cu in [1,16], refill in [1,64]");
+    if (blocks_per_cu == b->blocks_per_cu && refill == b->refill) return FORNOS_OK;
+    b->blocks_per_cu = blocks_per_cu;
+    b->refill = refill;
+    for (RayArgs& A : b->jobs) A.refill = refill;
+    HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * b->jobs.size(), hipMemcpyHostToDevice));
+    batch_drop_graph(b);
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_get_launch(const fornos_batch* b, uint32_t* blocks_per_cu, uint32_t* refill)
+{
+    if (!b) return set_error(FORNOS_INVALID_ARG, "fornos_batch_get_launch: null handle");
+    if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+    if (refill) *refill = b->refill;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_run(fornos_batch* b, float* ms)
+{
+    if (!b) return set_error(FORNOS_INVALID_ARG, "fornos_batch_run: null handle");
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const fornos_status st = batch_enqueue(b, b->jobs, b->djobs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    Timer tm(b->stream);
+    if (fornos_status st = tm.start()) return st;
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    if (fornos_status st = tm.stop(ms)) return st;
+    for (fornos_solver* s : b->solvers) s->offset = s->work_total;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* best_bpc, uint32_t* best_refill,
+                                    float* best_ms)
+{
+    if (!b || !(fraction > 0.0f) || fraction > 1.0f)
+        return set_error(FORNOS_INVALID_ARG, "fornos_batch_autotune: fraction must be in (0, 1]");
+    // a prefix of each job's texels (row-major: a band of the map)
+    std::vector<RayArgs> sample = b->jobs;
+    for (RayArgs& A : sample) A.ntex = std::max<uint32_t>(std::min<uint32_t>(A.ntex, 1), (uint32_t)(A.ntex * fraction));
+    RayArgs* dsample = nullptr;
+    if (fornos_status st = dev_alloc(&dsample, sample.size())) return st;
+    struct Free { RayArgs* p; ~Free() { dev_free(p); } } guard{dsample};
+    const uint32_t keep_bpc = b->blocks_per_cu, keep_refill = b->refill;
+    float best = FLT_MAX;
+    uint32_t bb = keep_bpc, br = keep_refill;
+    const uint32_t bpcs[] = {4, 6, 8, 12}, refills[] = {8, 16, 32};
+    for (uint32_t bpc : bpcs)
+        for (uint32_t rf : refills) {
+            b->blocks_per_cu = bpc;
+            for (RayArgs& A : sample) A.refill = rf;
+            HIP_TRY(hipMemcpy(dsample, sample.data(), sizeof(RayArgs) * sample.size(), hipMemcpyHostToDevice));
+            float ms = 0;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
+                Timer tm(b->stream);
+                if (fornos_status st = tm.start()) return st;
+                if (fornos_status st = batch_enqueue(b, sample, dsample)) return st;
+                if (fornos_status st = tm.stop(&ms)) return st;
+            }
+            if (ms < best) { best = ms; bb = bpc; br = rf; }
+        }
+    b->blocks_per_cu = keep_bpc;
+    b->refill = keep_refill;
+    if (fornos_status st = fornos_batch_set_launch(b, bb, br)) return st;
+    batch_drop_graph(b);
+    if (best_bpc) *best_bpc = bb;
+    if (best_refill) *best_refill = br;
+    if (best_ms) *best_ms = best;
+    return FORNOS_OK;
+}
+
+void fornos_batch_destroy(fornos_batch* b) { delete b; }
 
 }  // extern "C"

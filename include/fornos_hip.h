@@ -211,6 +211,30 @@ fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
                                        uint64_t *tri_tests, uint64_t *rays);
 void fornos_solver_destroy(fornos_solver *s);
 
+/* ---- batched jobs -------------------------------------------------------- */
+/* Several AO passes (each an AO solver with its own scene, texels and sample
+ * count) run as one persistent k_rays launch over a device job table, plus
+ * one reduction per job, captured into a HIP graph.  No reference
+ * counterpart: the reference runs one solver step per UI frame
+ * (Src/fornos.cpp:239-251); each job's results equal its solver's
+ * fornos_solver_run_all bit for bit and are read with fornos_solver_read.
+ * The solvers must stay alive while the batch exists; their own streams must
+ * be idle when fornos_batch_run is called. */
+typedef struct fornos_batch fornos_batch;
+fornos_status fornos_batch_create(fornos_solver *const *solvers, uint32_t n, fornos_batch **out);
+/* Launch shape: persistent blocks of 256 lanes per CU (1..16) and the
+ * number of idle lanes that triggers a wave's refill (1..64). */
+fornos_status fornos_batch_set_launch(fornos_batch *b, uint32_t blocks_per_cu, uint32_t refill);
+fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_per_cu, uint32_t *refill);
+/* Autotuner: times every (blocks_per_cu, refill) candidate on the first
+ * `fraction` of each job's texels and keeps the fastest. */
+fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
+                                    uint32_t *refill, float *best_ms);
+/* Runs every job once (graph launch); *ms = wall time of the graph on the
+ * device (HIP events). */
+fornos_status fornos_batch_run(fornos_batch *b, float *ms);
+void fornos_batch_destroy(fornos_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
