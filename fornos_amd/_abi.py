@@ -110,6 +110,26 @@ _SIGS = [
     ("fornos_batch_autotune", None, [H, C.c_float, u32p, u32p, C.POINTER(C.c_float)]),
     ("fornos_batch_run", None, [H, C.POINTER(C.c_float)]),
     ("fornos_batch_destroy", "void", [H]),
+    # denoiser (include/fornos_denoise.h)
+    ("fornos_denoiser_create", None, [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(H)]),
+    ("fornos_denoiser_param_count", None, [H, C.POINTER(C.c_uint64)]),
+    ("fornos_denoiser_layer_count", None, [H, u32p]),
+    ("fornos_denoiser_layer", None, [H, C.c_uint32, u32p, u32p, u32p, u32p, C.POINTER(C.c_uint64)]),
+    ("fornos_denoiser_get_params", None, [H, P, C.c_uint64]),
+    ("fornos_denoiser_set_params", None, [H, P, C.c_uint64]),
+    ("fornos_denoiser_forward", None, [H, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.POINTER(C.c_float)]),
+    ("fornos_denoiser_backward", None, [H, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.POINTER(C.c_float)]),
+    ("fornos_denoiser_get_grads", None, [H, P, C.c_uint64]),
+    ("fornos_denoiser_adam", None, [H, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("fornos_denoiser_destroy", "void", [H]),
+    ("fornos_denoise_gather", None, [H, H, H, H, H, H, P, C.c_float, C.c_float, P]),
+    ("fornos_denoise_scatter", None, [H, P, P, P]),
+    ("fornos_denoise_check_tiles", None, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                          C.c_float, P]),
+    ("fornos_dn_conv3x3", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P,
+                                 C.c_uint32, C.c_int, P]),
+    ("fornos_dn_conv3x3_wgrad", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       P, C.c_uint32, P, P]),
 ]
 
 SYMBOLS = [s[0] for s in _SIGS]
@@ -131,6 +151,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FornosError(UNSUPPORTED, "%s is missing: run `python -m fornos_amd.build` first" % LIB_PATH)
+    # torch's ROCm wheel carries its own libamdhip64 with the same soname as
+    # ROCm's.  Whichever is loaded first serves the whole process, and torch
+    # cannot initialise its device on the other one.  Load torch's first when
+    # torch is installed, so device tensors (used by the denoiser for memory)
+    # and this library share one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, res, args in _SIGS:
         f = getattr(L, name)

@@ -133,6 +133,38 @@ class CompressedMapUV(_Handle):
         return dict(zip(["positions", "directions", "normals", "tangents", "bitangents"], outs))
 
 
+class TexelArrays:
+    """A texel set held in numpy arrays (the layout of CompressedMapUV,
+    Src/compute.h:245-259): positions, directions, normals, tangents,
+    bitangents [n, 3] float32 and indices [n] uint32 into a width x height
+    map.  Used for sub-sets of a map (windows, tiles)."""
+
+    def __init__(self, arrays, width, height):
+        self.a = {k: np.ascontiguousarray(arrays[k], np.float32)
+                  for k in ("positions", "directions", "normals", "tangents", "bitangents")}
+        self.a["indices"] = np.ascontiguousarray(arrays["indices"], np.uint32)
+        self.width, self.height = int(width), int(height)
+
+    @property
+    def count(self):
+        return len(self.a["indices"])
+
+    def texel_view(self):
+        a = self.a
+        return _abi.TexelView(self.count, self.width, self.height, ptr(a["positions"]), ptr(a["directions"]),
+                              ptr(a["normals"]), ptr(a["tangents"]), ptr(a["bitangents"]), ptr(a["indices"]))
+
+    def arrays(self):
+        return dict(self.a)
+
+    def subset(self, select, indices=None, width=None, height=None):
+        """The texels where `select` holds, optionally re-indexed into another map."""
+        d = {k: v[select] for k, v in self.a.items()}
+        if indices is not None:
+            d["indices"] = indices
+        return TexelArrays(d, width or self.width, height or self.height)
+
+
 def sample_table(sample_count, n_perm=SAMPLE_PERM_COUNT, seed=1234):
     """(table[n_perm*S, 3] float32, rand[n_perm, 2] uint32), Src/math.h:363-382."""
     t = np.empty((n_perm * sample_count, 3), np.float32)

@@ -22,7 +22,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 
 HOST_SOURCES = ["mesh.cpp", "texels.cpp", "bvh.cpp", "fornos.cpp"]
-HIP_SOURCES = ["kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "denoise.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
           "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]

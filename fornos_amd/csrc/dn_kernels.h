@@ -1,0 +1,497 @@
+// dn_kernels.h -- gfx950 kernels of the AO / bent-normal denoiser (a 2-D
+// U-Net over the baked map): 3x3 convolutions as implicit GEMMs on the bf16
+// matrix cores (v_mfma_f32_16x16x32_bf16, fp32 accumulation), their weight
+// gradients, 2x2 average pooling, the 1x1 output head with the L1 loss, and
+// Adam.
+//
+// Tensors are NHWC: pixel-major, channels contiguous, bf16 stored as uint16_t
+// bit patterns.  A 16-pixel x 32-channel MFMA operand is then one 16-byte
+// load per lane: lane l takes pixel (l & 15) and channels 8*(l >> 4) .. +7.
+// Every convolution input has a multiple of 32 channels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fornos {
+namespace dn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;  // 4 waves
+
+__device__ __host__ inline float bf2f(uint16_t u)
+{
+    return __builtin_bit_cast(float, (uint32_t)u << 16);
+}
+
+// round to nearest even; NaN stays NaN
+__device__ __host__ inline uint16_t f2bf(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline bool bf_pos(uint16_t u) { return !(u & 0x8000u) && u != 0; }
+
+__device__ inline bf16x8 load8(const uint16_t* p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+__device__ inline bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+// One 3x3, stride-1, zero-padded convolution.  Input channels [0, c1) come
+// from `a` (at half resolution, nearest-upsampled, when up_a), channels
+// [c1, c1 + c2) from `b`: the decoder's concat is never materialised.
+// Output channels [0, split) go to out (split channels per pixel), channels
+// [split, cout) to out2; a mask tensor zeroes an output where the mask value
+// is not positive (the ReLU derivative in the backward pass); acc2 adds into
+// out2 instead of overwriting it.  The same kernel computes the backward
+// pass with respect to the input, with weights flipped and transposed.
+struct ConvArgs {
+    const uint16_t* a;
+    const uint16_t* b;
+    const uint16_t* wts;    // [cout][9][c1 + c2]
+    const float* bias;      // [cout] or null
+    void* out;              // bf16 or fp32 [pixels][split]
+    void* out2;             // bf16 [pixels][cout - split]
+    const uint16_t* mask1;  // [pixels][split] or null
+    const uint16_t* mask2;  // [pixels][cout - split] or null
+    uint32_t c1, c2, cout, split, n, h, w;
+    int up_a, relu, out_f32, acc2;
+};
+
+template <int MT, int NT>
+__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const uint32_t hw = A.h * A.w, P = A.n * hw;
+    const uint32_t pbase = (blockIdx.x * 4 + wave) * (MT * 16);
+    const uint32_t cbase = blockIdx.y * (NT * 16);
+    const uint32_t cin = A.c1 + A.c2;
+    const uint32_t hh = A.h >> 1, wh = A.w >> 1;
+    int py[MT], px[MT], pn[MT];
+    bool pv[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const uint32_t p = pbase + m * 16 + r;
+        pv[m] = p < P;
+        const uint32_t q = pv[m] ? p : 0;
+        pn[m] = q / hw;
+        const uint32_t rem = q - pn[m] * hw;
+        py[m] = rem / A.w;
+        px[m] = rem - py[m] * A.w;
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0, 0, 0, 0};
+
+    for (int t = 0; t < 9; ++t) {
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        size_t ia[MT], ib[MT];
+        bool sv[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int y = py[m] + dy, x = px[m] + dx;
+            sv[m] = pv[m] && y >= 0 && y < (int)A.h && x >= 0 && x < (int)A.w;
+            const int yc = sv[m] ? y : 0, xc = sv[m] ? x : 0;
+            ia[m] = A.up_a ? (((size_t)pn[m] * hh + (yc >> 1)) * wh + (xc >> 1)) * A.c1
+                           : (((size_t)pn[m] * A.h + yc) * A.w + xc) * A.c1;
+            ib[m] = (((size_t)pn[m] * A.h + yc) * A.w + xc) * A.c2;
+        }
+        const uint16_t* wt = A.wts + (size_t)t * cin + kq * 8;
+        for (uint32_t c0 = 0; c0 < cin; c0 += 32) {
+            bf16x8 af[MT], bw[NT];
+            const bool from_a = c0 < A.c1;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const uint16_t* src = from_a ? A.a + ia[m] + c0 + kq * 8 : A.b + ib[m] + (c0 - A.c1) + kq * 8;
+                af[m] = sv[m] ? load8(src) : zero8();
+            }
+#pragma unroll
+            for (int j = 0; j < NT; ++j) bw[j] = load8(wt + (size_t)(cbase + j * 16 + r) * 9 * cin + c0);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bw[j], acc[m][j], 0, 0, 0);
+        }
+    }
+    // D[m][n]: lane holds rows 4*kq .. 4*kq + 3 of column r
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const uint32_t co = cbase + j * 16 + r;
+        const float bias = A.bias ? A.bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t p = pbase + m * 16 + kq * 4 + e;
+                if (p >= P) continue;
+                float v = acc[m][j][e] + bias;
+                if (A.relu) v = v > 0.0f ? v : 0.0f;
+                if (co < A.split) {
+                    const size_t o = (size_t)p * A.split + co;
+                    if (A.mask1 && !bf_pos(A.mask1[o])) v = 0.0f;
+                    if (A.out_f32)
+                        static_cast<float*>(A.out)[o] = v;
+                    else
+                        static_cast<uint16_t*>(A.out)[o] = f2bf(v);
+                } else {
+                    const uint32_t c2o = A.cout - A.split;
+                    const size_t o = (size_t)p * c2o + (co - A.split);
+                    if (A.mask2 && !bf_pos(A.mask2[o])) v = 0.0f;
+                    uint16_t* o2 = static_cast<uint16_t*>(A.out2);
+                    if (A.acc2) v += bf2f(o2[o]);
+                    o2[o] = f2bf(v);
+                }
+            }
+    }
+}
+
+// Weight (and bias) gradient of k_conv3x3:
+//   dw[co][t][ci] += sum_p g[p][co] * x_t[p][ci],  db[co] += sum_p g[p][co]
+// x_t is the layer input shifted by tap t, read with the same concat /
+// upsample rules as the forward pass.  A block owns a 64 x 64 (co, ci) tile
+// of one tap over a range of pixels; 32-pixel chunks of g and x are staged in
+// LDS transposed (channel-major) so each lane reads its 8 consecutive pixels
+// as one 16-byte operand (K = pixels).  Partial tiles are added with fp32
+// atomics into dw, which the caller zeroes.
+struct WgradArgs {
+    const uint16_t* a;
+    const uint16_t* b;
+    const uint16_t* g;  // [pixels][cout]
+    float* dw;          // [cout][9][c1 + c2]
+    float* db;          // [cout] or null
+    uint32_t c1, c2, cout, n, h, w;
+    int up_a;
+    uint32_t ci_tiles, co_tiles, chunk_per_block;  // chunks of 32 pixels per block
+};
+
+constexpr int kWChunk = 32;
+constexpr int kWPad = kWChunk + 8;
+
+__global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs A)
+{
+    __shared__ uint16_t sg[64 * kWPad];
+    __shared__ uint16_t sx[64 * kWPad];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    uint32_t bid = blockIdx.y;
+    const uint32_t ct = bid % A.ci_tiles;
+    bid /= A.ci_tiles;
+    const uint32_t ot = bid % A.co_tiles;
+    const int t = (int)(bid / A.co_tiles);
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const uint32_t cin = A.c1 + A.c2;
+    const uint32_t ci0 = ct * 64, co0 = ot * 64;
+    const uint32_t hw = A.h * A.w, P = A.n * hw;
+    const uint32_t hh = A.h >> 1, wh = A.w >> 1;
+    const uint32_t p_begin = blockIdx.x * A.chunk_per_block * kWChunk;
+    const uint32_t p_end = min(P, p_begin + A.chunk_per_block * kWChunk);
+    const bool do_bias = A.db && t == 4 && ct == 0;
+
+    // loader role: pixel lp of the chunk, channels lc .. lc + 7 of the tile
+    const int lp = tid >> 3, lc = (tid & 7) * 8;
+    // compute role: wave -> (co half, ci half) of the tile
+    const int wco = (wave >> 1) * 32, wci = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+    float bsum = 0.0f;
+
+    for (uint32_t p0 = p_begin; p0 < p_end; p0 += kWChunk) {
+        const uint32_t p = p0 + lp;
+        uint4 gv = make_uint4(0, 0, 0, 0), xv = make_uint4(0, 0, 0, 0);
+        if (p < p_end) {
+            const uint32_t co = co0 + lc;
+            if (co < A.cout) gv = *reinterpret_cast<const uint4*>(A.g + (size_t)p * A.cout + co);
+            const uint32_t ci = ci0 + lc;
+            const uint32_t pn = p / hw, rem = p - pn * hw;
+            const int y = (int)(rem / A.w) + dy, x = (int)(rem % A.w) + dx;
+            if (ci < cin && y >= 0 && y < (int)A.h && x >= 0 && x < (int)A.w) {
+                const uint16_t* src;
+                if (ci < A.c1)
+                    src = A.up_a ? A.a + (((size_t)pn * hh + (y >> 1)) * wh + (x >> 1)) * A.c1 + ci
+                                 : A.a + (((size_t)pn * A.h + y) * A.w + x) * A.c1 + ci;
+                else
+                    src = A.b + (((size_t)pn * A.h + y) * A.w + x) * A.c2 + (ci - A.c1);
+                xv = *reinterpret_cast<const uint4*>(src);
+            }
+        }
+        __syncthreads();  // the previous chunk's reads are done
+        const uint16_t* g8 = reinterpret_cast<const uint16_t*>(&gv);
+        const uint16_t* x8 = reinterpret_cast<const uint16_t*>(&xv);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sg[(lc + j) * kWPad + lp] = g8[j];
+            sx[(lc + j) * kWPad + lp] = x8[j];
+        }
+        __syncthreads();
+        if (do_bias && tid < 64) {
+            for (int q = 0; q < kWChunk; ++q) bsum += bf2f(sg[tid * kWPad + q]);
+        }
+        bf16x8 af[2], bx[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) af[i] = load8(&sg[(wco + i * 16 + r) * kWPad + kq * 8]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bx[j] = load8(&sx[(wci + j * 16 + r) * kWPad + kq * 8]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bx[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t co = co0 + wco + i * 16 + kq * 4 + e;
+                const uint32_t ci = ci0 + wci + j * 16 + r;
+                if (co < A.cout && ci < cin) atomicAdd(&A.dw[((size_t)co * 9 + t) * cin + ci], acc[i][j][e]);
+            }
+    if (do_bias && tid < 64 && co0 + tid < A.cout) atomicAdd(&A.db[co0 + tid], bsum);
+}
+
+// 2x2 average pooling, 8 channels per thread: out[n][y][x] = mean of the
+// four input pixels (h, w even).
+__global__ void k_pool(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t h, uint32_t w, uint32_t c)
+{
+    const uint32_t ho = h >> 1, wo = w >> 1, c8 = c >> 3;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * ho * wo * c8) return;
+    const uint32_t cc = (uint32_t)(i % c8) * 8;
+    size_t q = i / c8;
+    const uint32_t x = q % wo;
+    q /= wo;
+    const uint32_t y = q % ho;
+    const uint32_t b = (uint32_t)(q / ho);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        const size_t src = (((size_t)b * h + 2 * y + (k >> 1)) * w + 2 * x + (k & 1)) * c + cc;
+        const uint4 v = *reinterpret_cast<const uint4*>(in + src);
+        const uint16_t* v8 = reinterpret_cast<const uint16_t*>(&v);
+        for (int j = 0; j < 8; ++j) s[j] += bf2f(v8[j]);
+    }
+    uint4 o;
+    uint16_t* o8 = reinterpret_cast<uint16_t*>(&o);
+    for (int j = 0; j < 8; ++j) o8[j] = f2bf(s[j] * 0.25f);
+    *reinterpret_cast<uint4*>(out + (((size_t)b * ho + y) * wo + x) * c + cc) = o;
+}
+
+// Backward of k_pool into the gradient of its input, which is the ReLU
+// output `act` of a convolution: gin += 0.25 * gout (upsampled), where act > 0.
+__global__ void k_pool_back(const uint16_t* gout, const uint16_t* act, uint16_t* gin, uint32_t n, uint32_t h,
+                            uint32_t w, uint32_t c)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c8 = c >> 3;
+    if (i >= (size_t)n * h * w * c8) return;
+    const uint32_t cc = (uint32_t)(i % c8) * 8;
+    size_t q = i / c8;
+    const uint32_t x = q % w;
+    q /= w;
+    const uint32_t y = q % h;
+    const uint32_t b = (uint32_t)(q / h);
+    const size_t o = (((size_t)b * h + y) * w + x) * c + cc;
+    const size_t s = (((size_t)b * (h >> 1) + (y >> 1)) * (w >> 1) + (x >> 1)) * c + cc;
+    const uint4 gv = *reinterpret_cast<const uint4*>(gout + s);
+    const uint4 av = *reinterpret_cast<const uint4*>(act + o);
+    uint4 iv = *reinterpret_cast<const uint4*>(gin + o);
+    const uint16_t* g8 = reinterpret_cast<const uint16_t*>(&gv);
+    const uint16_t* a8 = reinterpret_cast<const uint16_t*>(&av);
+    uint16_t* i8 = reinterpret_cast<uint16_t*>(&iv);
+    for (int j = 0; j < 8; ++j)
+        if (bf_pos(a8[j])) i8[j] = f2bf(bf2f(i8[j]) + 0.25f * bf2f(g8[j]));
+    *reinterpret_cast<uint4*>(gin + o) = iv;
+}
+
+// Backward of the nearest upsampling folded into k_conv3x3's `a` input:
+// gsmall[n][y][x] = sum of the 2x2 full-resolution gradients, where the
+// half-resolution activation act > 0.  (h, w: the half resolution.)
+__global__ void k_up_back(const uint16_t* gbig, const uint16_t* act, uint16_t* gsmall, uint32_t n, uint32_t h,
+                          uint32_t w, uint32_t c)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c8 = c >> 3;
+    if (i >= (size_t)n * h * w * c8) return;
+    const uint32_t cc = (uint32_t)(i % c8) * 8;
+    size_t q = i / c8;
+    const uint32_t x = q % w;
+    q /= w;
+    const uint32_t y = q % h;
+    const uint32_t b = (uint32_t)(q / h);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        const size_t src = (((size_t)b * 2 * h + 2 * y + (k >> 1)) * 2 * w + 2 * x + (k & 1)) * c + cc;
+        const uint4 v = *reinterpret_cast<const uint4*>(gbig + src);
+        const uint16_t* v8 = reinterpret_cast<const uint16_t*>(&v);
+        for (int j = 0; j < 8; ++j) s[j] += bf2f(v8[j]);
+    }
+    const size_t o = (((size_t)b * h + y) * w + x) * c + cc;
+    const uint4 av = *reinterpret_cast<const uint4*>(act + o);
+    const uint16_t* a8 = reinterpret_cast<const uint16_t*>(&av);
+    uint4 ov;
+    uint16_t* o8 = reinterpret_cast<uint16_t*>(&ov);
+    for (int j = 0; j < 8; ++j) o8[j] = bf_pos(a8[j]) ? f2bf(s[j]) : (uint16_t)0;
+    *reinterpret_cast<uint4*>(gsmall + o) = ov;
+}
+
+// Network input: fp32 [pixels][cin] -> bf16 [pixels][32], channels past cin zero.
+__global__ void k_pack_input(const float* in, uint16_t* out, size_t pixels, uint32_t cin)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels * 32) return;
+    const uint32_t c = (uint32_t)(i & 31);
+    const size_t p = i >> 5;
+    out[i] = c < cin ? f2bf(in[p * cin + c]) : (uint16_t)0;
+}
+
+// 1x1 head: out[p][o] = res[p][o] + hb[o] + sum_c hw[o][c] * y[p][c]
+// (o < 4, c < 32; res = the noisy AO and bent normal, input channels 0..3).
+constexpr int kHeadIn = 32;
+constexpr int kOut = 4;
+
+__global__ void k_head(const uint16_t* y, const float* in, uint32_t cin, const float* hw, const float* hb, float* out,
+                       size_t pixels)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float s[kOut];
+    for (int o = 0; o < kOut; ++o) s[o] = in[p * cin + o] + hb[o];
+    for (int c = 0; c < kHeadIn; c += 8) {
+        const uint4 v = *reinterpret_cast<const uint4*>(y + p * kHeadIn + c);
+        const uint16_t* v8 = reinterpret_cast<const uint16_t*>(&v);
+        for (int j = 0; j < 8; ++j) {
+            const float yv = bf2f(v8[j]);
+            for (int o = 0; o < kOut; ++o) s[o] += hw[o * kHeadIn + c + j] * yv;
+        }
+    }
+    for (int o = 0; o < kOut; ++o) out[p * kOut + o] = s[o];
+}
+
+// Masked L1 loss and the backward of the head.  valid = input channel
+// `mask_ch` > 0.5; L = (1/n_valid) sum_valid sum_o |out - target|.
+// Writes gy = dL/dy (bf16, zero where y <= 0) and accumulates the head's
+// weight and bias gradients and the loss: grid-stride over pixels, sums kept
+// per lane, reduced per wave, one fp32 atomic per wave and value.
+__device__ inline float wave_sum(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_head_back(const uint16_t* y, const float* in, uint32_t cin,
+                                                        uint32_t mask_ch, const float* out, const float* target,
+                                                        const float* hw, float inv_valid, uint16_t* gy,
+                                                        float* ghw, float* ghb, float* loss, size_t pixels)
+{
+    float gw[kOut][kHeadIn], gb[kOut], l = 0.0f;
+    for (int o = 0; o < kOut; ++o) {
+        gb[o] = 0.0f;
+        for (int c = 0; c < kHeadIn; ++c) gw[o][c] = 0.0f;
+    }
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x) {
+        const bool valid = in[p * cin + mask_ch] > 0.5f;
+        float d[kOut];
+        for (int o = 0; o < kOut; ++o) {
+            const float e = out[p * kOut + o] - target[p * kOut + o];
+            d[o] = 0.0f;
+            if (valid) {
+                l += fabsf(e);
+                d[o] = e > 0.0f ? inv_valid : (e < 0.0f ? -inv_valid : 0.0f);
+            }
+            gb[o] += d[o];
+        }
+        for (int c = 0; c < kHeadIn; ++c) {
+            const uint16_t u = y[p * kHeadIn + c];
+            const float yv = bf2f(u);
+            float gsum = 0.0f;
+            for (int o = 0; o < kOut; ++o) {
+                gsum += d[o] * hw[o * kHeadIn + c];
+                gw[o][c] += d[o] * yv;
+            }
+            gy[p * kHeadIn + c] = bf_pos(u) ? f2bf(gsum) : (uint16_t)0;
+        }
+    }
+    const bool lead = (threadIdx.x & 63) == 0;
+    const float lt = wave_sum(l);
+    if (lead) atomicAdd(loss, lt * inv_valid);
+    for (int o = 0; o < kOut; ++o) {
+        const float bt = wave_sum(gb[o]);
+        if (lead) atomicAdd(&ghb[o], bt);
+        for (int c = 0; c < kHeadIn; ++c) {
+            const float wt = wave_sum(gw[o][c]);
+            if (lead) atomicAdd(&ghw[o * kHeadIn + c], wt);
+        }
+    }
+}
+
+// Number of masked pixels (input channel mask_ch > 0.5).
+__global__ void k_count_valid(const float* in, uint32_t cin, uint32_t mask_ch, size_t pixels, uint32_t* count)
+{
+    uint32_t c = 0;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x)
+        c += in[p * cin + mask_ch] > 0.5f;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(count, c);
+}
+
+// Adam on fp32 master weights; refreshes the bf16 copy.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, size_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+// fp32 -> bf16 copy of weights
+__global__ void k_to_bf16(const float* w, uint16_t* wb, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = f2bf(w[i]);
+}
+
+// Weights of the input-gradient convolution: wt[ci][8 - t][co] = w[co][t][ci].
+__global__ void k_flip_weights(const uint16_t* w, uint16_t* wt, uint32_t cout, uint32_t cin)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)cout * 9 * cin) return;
+    const uint32_t ci = (uint32_t)(i % cin);
+    const uint32_t t = (uint32_t)((i / cin) % 9);
+    const uint32_t co = (uint32_t)(i / ((size_t)cin * 9));
+    wt[((size_t)ci * 9 + (8 - t)) * cout + co] = w[i];
+}
+
+__global__ void k_zero(float* p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace dn
+}  // namespace fornos

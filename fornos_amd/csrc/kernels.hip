@@ -22,6 +22,8 @@
 #include "mapping.h"
 #include "texel.h"
 #include "raster.h"
+#include "hiphost.h"
+#include "../../include/fornos_denoise.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -30,81 +32,6 @@
 using namespace fornos;
 
 
-namespace {
-
-// ---------------------------------------------------------------------------
-// Host side
-// ---------------------------------------------------------------------------
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess) {                                                           \
-            return set_error(e_ == hipErrorOutOfMemory ? FORNOS_OOM : FORNOS_HIP_ERROR,   \
-                             std::string(#expr) + ": " + hipGetErrorString(e_));          \
-        }                                                                                 \
-    } while (0)
-
-template <typename T>
-fornos_status dev_alloc(T** p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return set_error(e == hipErrorOutOfMemory ? FORNOS_OOM : FORNOS_HIP_ERROR,
-                         std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
-                             " bytes): " + hipGetErrorString(e));
-    }
-    return FORNOS_OK;
-}
-
-void dev_free(void* p)
-{
-    if (p) (void)hipFree(p);
-}
-
-fornos_status require_device()
-{
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        (void)hipGetLastError();
-        return set_error(FORNOS_UNSUPPORTED, "no HIP device is visible");
-    }
-    return FORNOS_OK;
-}
-
-struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
-    hipStream_t s;
-    explicit Timer(hipStream_t st) : s(st) {}
-    fornos_status start()
-    {
-        HIP_TRY(hipEventCreate(&a));
-        HIP_TRY(hipEventCreate(&b));
-        HIP_TRY(hipEventRecord(a, s));
-        return FORNOS_OK;
-    }
-    fornos_status stop(float* ms)
-    {
-        HIP_TRY(hipEventRecord(b, s));
-        HIP_TRY(hipEventSynchronize(b));
-        float v = 0;
-        HIP_TRY(hipEventElapsedTime(&v, a, b));
-        if (ms) *ms = v;
-        return FORNOS_OK;
-    }
-    ~Timer()
-    {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-};
-
-uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
-}  // namespace
 
 struct fornos_scene {
     int device = 0;
@@ -1000,5 +927,171 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
 }
 
 void fornos_batch_destroy(fornos_batch* b) { delete b; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Denoiser input / output on the texel set (include/fornos_denoise.h)
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ inline float finite_or0(float v) { return isfinite(v) ? v : 0.0f; }
+
+struct GatherArgs {
+    const uint32_t* indices;
+    const float *ao, *bn, *pos, *nrm, *hgt;
+    float* image;
+    uint32_t n, npix;
+    float cx, cy, cz, inv_extent, dist_scale;
+};
+
+__global__ void k_dn_gather(GatherArgs A)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.n) return;
+    const uint32_t pix = A.indices[q];
+    if (pix >= A.npix) return;
+    float* o = A.image + (size_t)pix * FORNOS_DN_IN;
+    o[0] = finite_or0(A.ao[q]);
+    for (int k = 0; k < 3; ++k) {
+        o[1 + k] = finite_or0(A.bn[3 * (size_t)q + k]);
+        o[7 + k] = finite_or0(A.nrm[3 * (size_t)q + k]);
+    }
+    o[4] = finite_or0((A.pos[3 * (size_t)q + 0] - A.cx) * A.inv_extent);
+    o[5] = finite_or0((A.pos[3 * (size_t)q + 1] - A.cy) * A.inv_extent);
+    o[6] = finite_or0((A.pos[3 * (size_t)q + 2] - A.cz) * A.inv_extent);
+    o[10] = finite_or0(A.hgt[q] * A.dist_scale);
+    o[FORNOS_DN_MASK_CHANNEL] = 1.0f;
+}
+
+__global__ void k_dn_fill(float* p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ao[q] = image AO clamped to [0, 1]; bn[q] = normalize(image bent normal)
+__global__ void k_dn_scatter(const uint32_t* indices, const float* image, uint32_t n, uint32_t npix, float* ao,
+                             float* bn)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t pix = indices[q];
+    if (pix >= npix) return;
+    const float* v = image + (size_t)pix * FORNOS_DN_OUT;
+    ao[q] = fminf(fmaxf(v[0], 0.0f), 1.0f);
+    const float x = v[1], y = v[2], z = v[3];
+    const float l = sqrtf(x * x + y * y + z * z);
+    const float s = l > 0.0f ? 1.0f / l : 0.0f;
+    bn[3 * (size_t)q + 0] = x * s;
+    bn[3 * (size_t)q + 1] = y * s;
+    bn[3 * (size_t)q + 2] = z * s;
+}
+
+// Per tile of tile x tile pixels: the mean of (noisy AO - denoised AO) over
+// the tile's texels against its standard error under the binomial noise of
+// an S-sample estimate, var = p (1 - p) / S per texel.  A tile fails when
+// |mean| > z * sqrt(sum var) / count + slack.
+__global__ void k_dn_tiles(const float* input, const float* output, uint32_t w, uint32_t h, uint32_t tile,
+                           uint32_t samples, float z, float slack, uint32_t* fail)
+{
+    const uint32_t tx = blockIdx.x, ty = blockIdx.y;
+    float sum = 0.0f, var = 0.0f, cnt = 0.0f;
+    for (uint32_t i = threadIdx.x; i < tile * tile; i += blockDim.x) {
+        const uint32_t x = tx * tile + i % tile, y = ty * tile + i / tile;
+        if (x >= w || y >= h) continue;
+        const size_t p = (size_t)y * w + x;
+        if (!(input[p * FORNOS_DN_IN + FORNOS_DN_MASK_CHANNEL] > 0.5f)) continue;
+        const float d = fminf(fmaxf(output[p * FORNOS_DN_OUT], 0.0f), 1.0f);
+        sum += input[p * FORNOS_DN_IN] - d;
+        var += d * (1.0f - d) / (float)samples;
+        cnt += 1.0f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        var += __shfl_xor(var, off);
+        cnt += __shfl_xor(cnt, off);
+    }
+    __shared__ float red[3][4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = sum;
+        red[1][wv] = var;
+        red[2][wv] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float S = 0, V = 0, N = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            S += red[0][k];
+            V += red[1][k];
+            N += red[2][k];
+        }
+        const bool bad = N > 0.0f && fabsf(S / N) > z * sqrtf(V) / N + slack;
+        fail[(size_t)ty * gridDim.x + tx] = bad ? 1u : 0u;
+    }
+}
+
+fornos_status dn_solver_ok(const fornos_solver* s, const fornos_meshmap* mm, int kind, const char* what)
+{
+    if (!s || s->mm != mm || s->kind != kind)
+        return set_error(FORNOS_INVALID_ARG, std::string("fornos_denoise_gather: ") + what +
+                                                 " must be a solver of that kind on the same mesh mapping");
+    return FORNOS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fornos_status fornos_denoise_gather(const fornos_meshmap* mm, const fornos_solver* ao, const fornos_solver* bn,
+                                    const fornos_solver* position, const fornos_solver* normals,
+                                    const fornos_solver* height, const float* center, float inv_extent,
+                                    float dist_scale, float* image)
+{
+    if (!mm || !center || !image) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: null argument");
+    if (!mm->indices) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: the texel set has no map indices");
+    if (fornos_status s = dn_solver_ok(ao, mm, FORNOS_SOLVER_AO, "ao")) return s;
+    if (fornos_status s = dn_solver_ok(bn, mm, FORNOS_SOLVER_BENT_NORMALS, "bn")) return s;
+    if (fornos_status s = dn_solver_ok(position, mm, FORNOS_SOLVER_POSITION, "position")) return s;
+    if (fornos_status s = dn_solver_ok(normals, mm, FORNOS_SOLVER_NORMALS, "normals")) return s;
+    if (fornos_status s = dn_solver_ok(height, mm, FORNOS_SOLVER_HEIGHT, "height")) return s;
+    hipStream_t st = mm->sc->stream;
+    const size_t npix = (size_t)mm->width * mm->height;
+    hipLaunchKernelGGL(k_dn_fill, dim3(div_up(npix * FORNOS_DN_IN, kBlock)), dim3(kBlock), 0, st, image,
+                       npix * FORNOS_DN_IN);
+    GatherArgs A{mm->indices, ao->out, bn->out, position->out, normals->out, height->out, image, mm->n,
+                 (uint32_t)npix, center[0], center[1], center[2], inv_extent, dist_scale};
+    if (mm->n) hipLaunchKernelGGL(k_dn_gather, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st, A);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return FORNOS_OK;
+}
+
+fornos_status fornos_denoise_scatter(const fornos_meshmap* mm, const float* image, float* ao, float* bn)
+{
+    if (!mm || !image || !ao || !bn) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_scatter: null argument");
+    if (!mm->indices) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_scatter: the texel set has no map indices");
+    hipStream_t st = mm->sc->stream;
+    if (mm->n)
+        hipLaunchKernelGGL(k_dn_scatter, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)mm->indices,
+                           image, mm->n, mm->width * mm->height, ao, bn);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return FORNOS_OK;
+}
+
+fornos_status fornos_denoise_check_tiles(const float* input, const float* output, uint32_t width, uint32_t height,
+                                         uint32_t tile, uint32_t samples, float z, float slack, uint32_t* fail)
+{
+    if (!input || !output || !fail || tile == 0 || samples == 0 || width == 0 || height == 0)
+        return set_error(FORNOS_INVALID_ARG, "fornos_denoise_check_tiles: bad argument");
+    const dim3 grid(div_up(width, tile), div_up(height, tile));
+    hipLaunchKernelGGL(k_dn_tiles, grid, dim3(kBlock), 0, nullptr, input, output, width, height, tile, samples, z,
+                       slack, fail);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return FORNOS_OK;
+}
 
 }  // extern "C"
