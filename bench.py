@@ -68,6 +68,32 @@ def cpu_baseline(prep, mm, solver, budget_s=15.0):
                       % (texels, 256, S, rays, t)}
 
 
+def denoise_leg(prep):
+    """Denoised 16-sample bake (AO + bent normals through the U-Net, with the
+    per-tile fallback) against the plain 1024-sample bake of the same map:
+    wall times and the errors of the denoised maps against the plain ones."""
+    import numpy as np
+    from fornos_amd import denoise as dn
+    if not os.path.exists(dn.WEIGHTS):
+        return None
+    import torch
+    cfg = prep["cfg"]
+    spec = {"seed": 0, "family": "heightfield", "low": cfg["low"], "high": cfg["high"], "cull": cfg["cull"],
+            "ao": cfg["maps"]["ao"]}
+    job = dn.MeshJob(spec, cfg["size"], (prep["low"], prep["dirs"], prep["high"], prep["scene"]))
+    net = dn.Denoiser.load(dn.WEIGHTS)
+    net.forward(torch.zeros((1, cfg["size"], cfg["size"], dn.IN), dtype=torch.float32, device="cuda"))
+    ra, rb, tp, mm = dn.plain_bake(job)
+    _, tidx = mm.read()
+    hit = tidx != 0xFFFFFFFF
+    da, db, td, info = dn.denoised_bake(job, net)
+    ae, be = dn.errors(da, db, ra, rb, hit)
+    print("# denoise: plain %.2f s, denoised %.3f s, AO MAE %.4f, BN %.2f deg" % (tp, td, ae, be),
+          file=sys.stderr, flush=True)
+    return {"plain_1024_s": tp, "denoised_16_s": td, "speedup": tp / td, "ao_mae": ae, "bn_deg": be,
+            "network_ms": info["network_ms"], "fallback_tiles": info["fallback_tiles"], "tiles": info["tiles"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +101,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bake", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (torch), the known-size kernel "
@@ -170,6 +197,7 @@ def main():
             s.getResults()
         bake = {"seconds": time.perf_counter() - t0, "kernel_ms": parts}
 
+    denoise = None if args.no_denoise else denoise_leg(prep)
     cpu = None if args.no_cpu else cpu_baseline(prep, mm, ao)
     line = {
         "metric": "AO rays/s", "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
@@ -180,7 +208,7 @@ def main():
                                   p["maxDistance"]),
                    "texels": n, "rays_per_step": rays_per_step,
                    "bvh": "reference binary tree (exact reachability) + 4-wide SAH culling tree"},
-        "roofline": roofline, "cpu_baseline": cpu, "mapping_ms": map_ms, "bake": bake,
+        "roofline": roofline, "cpu_baseline": cpu, "mapping_ms": map_ms, "bake": bake, "denoise": denoise,
         "texels_per_s": live / (ms_per_step * 1e-3),
     }
     print(json.dumps(line), flush=True)

@@ -102,12 +102,15 @@ _SIGS = [
     ("fornos_solver_read", None, [H, P, C.c_size_t]),
     ("fornos_solver_read_counts", None, [H, P]),
     ("fornos_solver_last_timing", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_float), u32p]),
-    ("fornos_solver_count_work", None, [H, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("fornos_solver_count_work", None, [H, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
+    ("fornos_solver_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("fornos_solver_get_launch", None, [H, u32p, u32p, u32p]),
     ("fornos_solver_destroy", "void", [H]),
     ("fornos_batch_create", None, [C.POINTER(H), C.c_uint32, C.POINTER(H)]),
-    ("fornos_batch_set_launch", None, [H, C.c_uint32, C.c_uint32]),
-    ("fornos_batch_get_launch", None, [H, u32p, u32p]),
-    ("fornos_batch_autotune", None, [H, C.c_float, u32p, u32p, C.POINTER(C.c_float)]),
+    ("fornos_batch_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("fornos_batch_get_launch", None, [H, u32p, u32p, u32p]),
+    ("fornos_batch_autotune", None, [H, C.c_float, u32p, u32p, u32p, C.POINTER(C.c_float)]),
     ("fornos_batch_run", None, [H, C.POINTER(C.c_float)]),
     ("fornos_batch_destroy", "void", [H]),
     # denoiser (include/fornos_denoise.h)

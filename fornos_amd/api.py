@@ -333,9 +333,17 @@ class Solver(_Handle):
         return a.value, b.value, n.value
 
     def count_work(self):
-        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"node_tests": a.value, "tri_tests": b.value, "rays": c.value}
+        a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"node_tests": a.value, "tri_tests": b.value, "rays": c.value, "stack_overflows": d.value}
+
+    def set_launch(self, blocks_per_cu=8, refill=16, stack=24):
+        check(lib().fornos_solver_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
+
+    def get_launch(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().fornos_solver_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
 
 class HeightSolver(Solver):
@@ -393,20 +401,21 @@ class Batch(_Handle):
         arr = (C.c_void_p * len(self.solvers))(*[s.handle.value for s in self.solvers])
         check(lib().fornos_batch_create(arr, len(self.solvers), C.byref(self._h)))
 
-    def set_launch(self, blocks_per_cu, refill):
-        check(lib().fornos_batch_set_launch(self._h, int(blocks_per_cu), int(refill)))
+    def set_launch(self, blocks_per_cu, refill, stack=24):
+        check(lib().fornos_batch_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
 
     def get_launch(self):
-        a, b = C.c_uint32(), C.c_uint32()
-        check(lib().fornos_batch_get_launch(self._h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().fornos_batch_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def autotune(self, fraction=0.05):
-        """Times every launch-shape candidate on a fraction of each job's texels,
-        keeps the fastest; returns (blocks_per_cu, refill, ms)."""
-        a, b, ms = C.c_uint32(), C.c_uint32(), C.c_float()
-        check(lib().fornos_batch_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(ms)))
-        return a.value, b.value, ms.value
+        """Times launch shapes on a fraction of each job's texels, keeps the
+        fastest; returns (blocks_per_cu, refill, stack, ms)."""
+        a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().fornos_batch_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(c),
+                                          C.byref(ms)))
+        return a.value, b.value, c.value, ms.value
 
     def run(self):
         ms = C.c_float()

@@ -115,16 +115,16 @@ def tune(batch, jobs, cache_path=None, fraction=0.05):
         with open(cache_path) as f:
             cache = json.load(f)
     if key in cache:
-        bpc, refill = cache[key]["blocks_per_cu"], cache[key]["refill"]
-        batch.set_launch(bpc, refill)
-        return {"blocks_per_cu": bpc, "refill": refill, "cached": True}
-    bpc, refill, ms = batch.autotune(fraction)
+        c = cache[key]
+        batch.set_launch(c["blocks_per_cu"], c["refill"], c["stack"])
+        return {"blocks_per_cu": c["blocks_per_cu"], "refill": c["refill"], "stack": c["stack"], "cached": True}
+    bpc, refill, stack, ms = batch.autotune(fraction)
     if cache_path:
-        cache[key] = {"blocks_per_cu": bpc, "refill": refill, "sample_ms": ms}
+        cache[key] = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "sample_ms": ms}
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
-    return {"blocks_per_cu": bpc, "refill": refill, "sample_ms": ms, "cached": False}
+    return {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "sample_ms": ms, "cached": False}
 
 
 def main(argv=None):

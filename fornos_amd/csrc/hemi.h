@@ -32,7 +32,7 @@ namespace {
 enum { kAO = 0, kBN = 1, kThick = 2 };
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
 constexpr uint32_t kRefill = 16;  // default: refill a wave once this many lanes are idle
-constexpr uint32_t kStack = 24;   // 4-wide traversal stack entries per lane
+constexpr uint32_t kStack = 24;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
 
 struct RayArgs {
     DevScene sc;
@@ -101,9 +101,10 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
     return 0;
 }
 
-// One node of the 4-wide culling tree (or one stack pop).
-template <int KIND, bool COUNT>
-__device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, float* stk_t, uint32_t& cn)
+// One node of the 4-wide culling tree (or one stack pop); STK stack entries.
+template <int KIND, bool COUNT, int STK>
+__device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, float* stk_t, uint32_t& cn,
+                                 uint32_t& nov)
 {
     const uint32_t tid = threadIdx.x;
     const float T = KIND == kThick ? fminf(L.best, A.maxd) : A.maxd;
@@ -148,7 +149,8 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
 #pragma unroll
     for (int k = 3; k >= 1; --k) {
         if (c[k] == kNoChild) continue;
-        if (L.sp == kStack) {  // redo the ray on the stackless binary culling tree
+        if (L.sp == STK) {  // redo the ray on the stackless binary culling tree
+            if (COUNT) ++nov;
             L.mode = kCullTree;
             L.node = 0;
             L.best = FLT_MAX;
@@ -214,11 +216,11 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
 // samples, counter and outputs).  A wave works on one job at a time -- the
 // job index is wave-uniform, so its parameters live in scalar registers --
 // and moves on once the job's counter is exhausted and its lanes are idle.
-template <int KIND, bool COUNT, bool BATCH>
+template <int KIND, bool COUNT, bool BATCH, int STK>
 __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __restrict__ jobs, uint32_t njobs)
 {
-    __shared__ uint32_t stk_node[kStack * kBlock];
-    __shared__ float stk_t[KIND == kThick ? kStack * kBlock : 1];
+    __shared__ uint32_t stk_node[STK * kBlock];
+    __shared__ float stk_t[KIND == kThick ? STK * kBlock : 1];
     uint32_t job = 0;
     RayArgs A = BATCH ? jobs[0] : A0;
     uint32_t total = A.ntex * A.S;
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
     L.mode = kWideTree;
     L.active = false;
     bool exhausted = false;
-    uint32_t cn = 0, ct = 0, nr = 0;
+    uint32_t cn = 0, ct = 0, nr = 0, nov = 0;
     for (;;) {
         // refill idle lanes, one atomic per wave
         const uint64_t idle = __ballot(!L.active);
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
         for (;;) {
             const bool walk = L.active && L.mode == kWideTree && (L.cur == kNoChild || !(L.cur & kLeafFlag));
             if (__ballot(walk) == 0) break;
-            if (walk) wide_step<KIND, COUNT>(A, L, stk_node, stk_t, cn);
+            if (walk) wide_step<KIND, COUNT, STK>(A, L, stk_node, stk_t, cn, nov);
         }
         // phase 2: leaves together; rays on a binary tree take one step
         if (L.active) {
@@ -301,13 +303,15 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
         }
     }
     if (COUNT) {
-        unsigned long long a = cn, b = ct, c = nr;
+        unsigned long long a = cn, b = ct, c = nr, o = nov;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_down(a, off);
             b += __shfl_down(b, off);
             c += __shfl_down(c, off);
+            o += __shfl_down(o, off);
         }
         if (lane == 0) {
+            atomicAdd(&A.stats[3], o);
             atomicAdd(&A.stats[0], a);
             atomicAdd(&A.stats[1], b);
             atomicAdd(&A.stats[2], c);

@@ -35,7 +35,7 @@ using namespace fornos;
 
 struct fornos_scene {
     int device = 0;
-    uint32_t persistent_blocks = 2048;  // k_rays grid: 8 blocks of 256 lanes per CU
+    uint32_t cus = 256;  // compute units of the scene's device
     hipStream_t stream = nullptr;
     DevScene d{};
     float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr, *cnodes = nullptr;
@@ -80,6 +80,7 @@ struct fornos_solver {
     unsigned long long* stats = nullptr;
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
+    uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
     uint32_t* counter = nullptr;       // k_rays work counter
     // per-launch events of the last timed run: (before k_rays, after k_rays, after k_reduce)
     std::vector<hipEvent_t> events;
@@ -103,6 +104,24 @@ struct fornos_solver {
 namespace {
 
 // Launch a hemisphere kernel over texels [t0, t1).
+// k_rays with a stack of 16, 24 or 32 entries per lane
+template <int KIND, bool COUNT, bool BATCH>
+void launch_rays(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs& A, const RayArgs* jobs, uint32_t nj)
+{
+    if (stack == 16)
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 16>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+    else if (stack == 32)
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 32>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+    else
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 24>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+}
+
+bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
+{
+    return blocks_per_cu >= 1 && blocks_per_cu <= 16 && refill >= 1 && refill <= 64 &&
+           (stack == 16 || stack == 24 || stack == 32);
+}
+
 template <int KIND>
 fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count)
 {
@@ -125,7 +144,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.mwords = mwords;
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
-    A.refill = kRefill;
+    A.refill = s->refill;
     ReduceArgs R;
     R.sc = sc->d;
     R.coords = mm->coords;
@@ -149,7 +168,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         const uint32_t nt = std::min(s->chunk_texels, t1 - c0);
         A.t0 = R.t0 = c0;
         A.ntex = R.ntex = nt;
-        const uint32_t blocks = std::min<uint32_t>(sc->persistent_blocks, div_up((uint64_t)nt * S, kBlock));
+        const uint32_t blocks = std::min<uint32_t>(sc->cus * s->blocks_per_cu, div_up((uint64_t)nt * S, kBlock));
         fill(s->counter, 1, 0);
         if (KIND == kAO) fill(s->counts + c0, nt, 0);
         if (KIND == kBN) fill(s->raybuf, (size_t)nt * mwords, 0);
@@ -161,11 +180,10 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
             for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
             HIP_TRY(hipEventRecord(ev[0], sc->stream));
         }
-        const RayArgs* no_jobs = nullptr;
         if (count)
-            hipLaunchKernelGGL((k_rays<KIND, true, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A, no_jobs, 0u);
+            launch_rays<KIND, true, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
         else
-            hipLaunchKernelGGL((k_rays<KIND, false, false>), dim3(blocks), dim3(kBlock), 0, sc->stream, A, no_jobs, 0u);
+            launch_rays<KIND, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
         if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
@@ -241,7 +259,7 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     HIP_TRY(hipGetDevice(&sc->device));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sc->device));
-    sc->persistent_blocks = (uint32_t)std::max(1, cus) * 8;
+    sc->cus = (uint32_t)std::max(1, cus);
     const auto t_begin = std::chrono::steady_clock::now();
     build_reference_bvh(*mesh, bp.max_tris_per_leaf, bp.max_depth, sc->bvh);
     const FlatBVH& B = sc->bvh;
@@ -718,17 +736,39 @@ fornos_status fornos_solver_read_counts(const fornos_solver* s, uint32_t* out)
     return FORNOS_OK;
 }
 
-fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays)
+fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays,
+                                       uint64_t* overflows)
 {
     if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: null handle");
     if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: not a hemisphere solver");
     if (fornos_status st = solver_launch(s, 0, s->mm->n, true)) return st;
-    unsigned long long h[3] = {0, 0, 0};
+    unsigned long long h[4] = {0, 0, 0, 0};
     HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
     HIP_TRY(hipMemcpy(h, s->stats, sizeof(h), hipMemcpyDeviceToHost));
     if (nodes) *nodes = h[0];
     if (tris) *tris = h[1];
     if (rays) *rays = h[2];
+    if (overflows) *overflows = h[3];
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_set_launch(fornos_solver* s, uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
+{
+    if (!s || !launch_shape_ok(blocks_per_cu, refill, stack))
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_set_launch: blocks_per_cu in [1,16], refill in [1,64], stack 16, 24 or 32");
+    s->blocks_per_cu = blocks_per_cu;
+    s->refill = refill;
+    s->stack = stack;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_get_launch(const fornos_solver* s, uint32_t* blocks_per_cu, uint32_t* refill,
+                                       uint32_t* stack)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_launch: null handle");
+    if (blocks_per_cu) *blocks_per_cu = s->blocks_per_cu;
+    if (refill) *refill = s->refill;
+    if (stack) *stack = s->stack;
     return FORNOS_OK;
 }
 
@@ -743,7 +783,7 @@ struct fornos_batch {
     RayArgs* djobs = nullptr;
     uint32_t* counters = nullptr;
     int cus = 256;
-    uint32_t blocks_per_cu = 8, refill = kRefill;
+    uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     ~fornos_batch()
@@ -771,7 +811,7 @@ fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, R
     uint64_t rays = 0;
     for (const RayArgs& J : jobs) rays += (uint64_t)J.ntex * J.S;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)b->cus * b->blocks_per_cu, div_up(rays, kBlock));
-    if (blocks) hipLaunchKernelGGL((k_rays<kAO, false, true>), dim3(blocks), dim3(kBlock), 0, st, jobs[0], djobs, nj);
+    if (blocks) launch_rays<kAO, false, true>(b->stack, blocks, st, jobs[0], djobs, nj);
     for (uint32_t j = 0; j < nj; ++j) {
         const fornos_solver* s = b->solvers[j];
         ReduceArgs R{};
@@ -842,24 +882,27 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
     return FORNOS_OK;
 }
 
-fornos_status fornos_batch_set_launch(fornos_batch* b, uint32_t blocks_per_cu, uint32_t refill)
+fornos_status fornos_batch_set_launch(fornos_batch* b, uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
 {
-    if (!b || blocks_per_cu < 1 || blocks_per_cu > 16 || refill < 1 || refill > 64)
-        return set_error(FORNOS_INVALID_ARG, "fornos_batch_set_launch: blocks_per_cu in [1,16], refill in [1,64]");
-    if (blocks_per_cu == b->blocks_per_cu && refill == b->refill) return FORNOS_OK;
+    if (!b || !launch_shape_ok(blocks_per_cu, refill, stack))
+        return set_error(FORNOS_INVALID_ARG, "fornos_batch_set_launch: blocks_per_cu in [1,16], refill in [1,64], stack 16, 24 or 32");
+    if (blocks_per_cu == b->blocks_per_cu && refill == b->refill && stack == b->stack) return FORNOS_OK;
     b->blocks_per_cu = blocks_per_cu;
     b->refill = refill;
+    b->stack = stack;
     for (RayArgs& A : b->jobs) A.refill = refill;
     HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * b->jobs.size(), hipMemcpyHostToDevice));
     batch_drop_graph(b);
     return FORNOS_OK;
 }
 
-fornos_status fornos_batch_get_launch(const fornos_batch* b, uint32_t* blocks_per_cu, uint32_t* refill)
+fornos_status fornos_batch_get_launch(const fornos_batch* b, uint32_t* blocks_per_cu, uint32_t* refill,
+                                      uint32_t* stack)
 {
     if (!b) return set_error(FORNOS_INVALID_ARG, "fornos_batch_get_launch: null handle");
     if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
     if (refill) *refill = b->refill;
+    if (stack) *stack = b->stack;
     return FORNOS_OK;
 }
 
@@ -888,7 +931,7 @@ fornos_status fornos_batch_run(fornos_batch* b, float* ms)
 }
 
 fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* best_bpc, uint32_t* best_refill,
-                                    float* best_ms)
+                                    uint32_t* best_stack, float* best_ms)
 {
     if (!b || !(fraction > 0.0f) || fraction > 1.0f)
         return set_error(FORNOS_INVALID_ARG, "fornos_batch_autotune: fraction must be in (0, 1]");
@@ -898,30 +941,46 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
     RayArgs* dsample = nullptr;
     if (fornos_status st = dev_alloc(&dsample, sample.size())) return st;
     struct Free { RayArgs* p; ~Free() { dev_free(p); } } guard{dsample};
-    const uint32_t keep_bpc = b->blocks_per_cu, keep_refill = b->refill;
-    float best = FLT_MAX;
-    uint32_t bb = keep_bpc, br = keep_refill;
-    const uint32_t bpcs[] = {4, 6, 8, 12}, refills[] = {8, 16, 32};
-    for (uint32_t bpc : bpcs)
-        for (uint32_t rf : refills) {
-            b->blocks_per_cu = bpc;
-            for (RayArgs& A : sample) A.refill = rf;
-            HIP_TRY(hipMemcpy(dsample, sample.data(), sizeof(RayArgs) * sample.size(), hipMemcpyHostToDevice));
-            float ms = 0;
-            for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
-                Timer tm(b->stream);
-                if (fornos_status st = tm.start()) return st;
-                if (fornos_status st = batch_enqueue(b, sample, dsample)) return st;
-                if (fornos_status st = tm.stop(&ms)) return st;
-            }
-            if (ms < best) { best = ms; bb = bpc; br = rf; }
+    const uint32_t keep[3] = {b->blocks_per_cu, b->refill, b->stack};
+    uint32_t cur[3] = {keep[0], keep[1], keep[2]};  // blocks per CU, refill, stack
+    auto time_shape = [&](const uint32_t* shape, float* ms) -> fornos_status {
+        b->blocks_per_cu = shape[0];
+        b->stack = shape[2];
+        for (RayArgs& A : sample) A.refill = shape[1];
+        HIP_TRY(hipMemcpy(dsample, sample.data(), sizeof(RayArgs) * sample.size(), hipMemcpyHostToDevice));
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
+            Timer tm(b->stream);
+            if (fornos_status st = tm.start()) return st;
+            if (fornos_status st = batch_enqueue(b, sample, dsample)) return st;
+            if (fornos_status st = tm.stop(ms)) return st;
         }
-    b->blocks_per_cu = keep_bpc;
-    b->refill = keep_refill;
-    if (fornos_status st = fornos_batch_set_launch(b, bb, br)) return st;
+        return FORNOS_OK;
+    };
+    // coordinate descent: stack depth, then blocks per CU, then refill threshold
+    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {8, 16, 32}, {16, 24, 32}};
+    float best = FLT_MAX;
+    for (int axis : {2, 0, 1}) {
+        uint32_t pick = cur[axis];
+        for (uint32_t v : cand[axis]) {
+            uint32_t shape[3] = {cur[0], cur[1], cur[2]};
+            shape[axis] = v;
+            float ms = 0;
+            if (fornos_status st = time_shape(shape, &ms)) return st;
+            if (ms < best) {
+                best = ms;
+                pick = v;
+            }
+        }
+        cur[axis] = pick;
+    }
+    b->blocks_per_cu = keep[0];
+    b->refill = keep[1];
+    b->stack = keep[2];
+    if (fornos_status st = fornos_batch_set_launch(b, cur[0], cur[1], cur[2])) return st;
     batch_drop_graph(b);
-    if (best_bpc) *best_bpc = bb;
-    if (best_refill) *best_refill = br;
+    if (best_bpc) *best_bpc = cur[0];
+    if (best_refill) *best_refill = cur[1];
+    if (best_stack) *best_stack = cur[2];
     if (best_ms) *best_ms = best;
     return FORNOS_OK;
 }

@@ -290,8 +290,9 @@ void gen_torus_knot(fornos_mesh& m, uint32_t nu, uint32_t nv, float amp, uint64_
             uint32_t i1 = (i + 1) % nu, j1 = (j + 1) % nv;
             uint32_t a = i * nv + j, b = i1 * nv + j, c = i1 * nv + j1, d = i * nv + j1;
             uint32_t ta = i * (nv + 1) + j, tb = (i + 1) * (nv + 1) + j, tc = tb + 1, td = ta + 1;
-            add_tri(m, a, b, c, ta, tb, tc, a, b, c);
-            add_tri(m, a, c, d, ta, tc, td, a, c, d);
+            // counter-clockwise seen from outside the tube: face normals point outward
+            add_tri(m, a, c, b, ta, tc, tb, a, c, b);
+            add_tri(m, a, d, c, ta, td, tc, a, d, c);
         }
     vertex_normals(m);
 }

@@ -130,15 +130,20 @@ class Denoiser(_Handle):
 class MeshJob:
     """A seeded mesh pair with its device scene and map-sized texel set."""
 
-    def __init__(self, spec, size=2048):
+    def __init__(self, spec, size=2048, meshes=None):
+        """meshes: an already prepared (low, dirs, high, scene), else built from spec."""
         self.spec = spec
         self.size = size
-        lo = Mesh.generate(*spec["low"])
-        hi = Mesh.generate(*spec["high"])
-        dirs = lo.copy()
-        dirs.computeVertexNormalsAggressive()
+        if meshes is None:
+            lo = Mesh.generate(*spec["low"])
+            hi = Mesh.generate(*spec["high"])
+            dirs = lo.copy()
+            dirs.computeVertexNormalsAggressive()
+            scene = Scene(hi)
+        else:
+            lo, dirs, hi, scene = meshes
         self.lo, self.dirs = lo, dirs
-        self.scene = Scene(hi)
+        self.scene = scene
         self.cull = spec["cull"]
         pos = hi.arrays()["positions"]
         a, b = pos.min(0), pos.max(0)
@@ -159,8 +164,13 @@ class MeshJob:
         local = ((y[sel] - y0) * w + (x[sel] - x0)).astype(np.uint32)
         return self.texels.subset(sel, local, w, w), sel
 
-    def mapping(self, texels):
-        mm = MeshMapping().init(texels, self.scene, self.cull)
+    def mapping(self, texels=None):
+        """Mesh mapping of a texel sub-set, or of the whole map (texel set
+        rasterised on the device, nothing uploaded)."""
+        if texels is None:
+            mm = MeshMapping().init_from_mesh(self.lo, self.dirs, self.size, self.size, self.scene, self.cull)
+        else:
+            mm = MeshMapping().init(texels, self.scene, self.cull)
         mm.run_all()
         return mm
 
@@ -209,7 +219,7 @@ def training_pair(job, x0, y0, w, seed):
 def plain_bake(job, S=HIGH_S, seed=7):
     """Mapping + AO + bent normals at S samples; host results and wall time."""
     t0 = time.perf_counter()
-    mm = job.mapping(job.texels)
+    mm = job.mapping()
     ao, bn = job.solvers(mm, S, seed)
     ao.run_all()
     bn.run_all()
@@ -225,7 +235,7 @@ def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True)
     torch = _torch()
     W = job.size
     t0 = time.perf_counter()
-    mm = job.mapping(job.texels)
+    mm = job.mapping()
     ao, bn = job.solvers(mm, LOW_S, seed)
     ao.run_all()
     bn.run_all()
@@ -298,9 +308,8 @@ def make_dataset(seeds, windows=2, wsize=512, size=2048, log=print):
             y0 = int(np.clip(q // size - wsize // 2, 0, size - wsize)) // 8 * 8
             inp, tgt = training_pair(job, x0, y0, wsize, 100 * spec["seed"] + wi)
             data.append((inp, tgt))
-        del job
         log("# data %d/%d seed %d %s %d hi tris: %.1f s" % (k + 1, len(seeds), spec["seed"], spec["family"],
-                                                            spec["high"][3], time.perf_counter() - t))
+                                                            job.high_tris, time.perf_counter() - t))
     torch.cuda.synchronize()
     return data
 
@@ -341,6 +350,44 @@ def train(net, data, steps=3000, batch=16, crop=128, lr=1e-3, seed=0, log=print)
     return hist
 
 
+def conv_flops(net, n, h, w):
+    """Algorithmic flops of one forward pass: 2 * 9 * cin * cout per output
+    pixel of every 3x3 layer (cin of the first layer = the 12 real input
+    channels) plus the 1x1 head."""
+    total = 0
+    for i, L in enumerate(net.layers()):
+        level = 0
+        # encoder layers 2l, 2l+1 sit at level l; decoder levels run L-2 .. 0
+        if i < 2 * net.levels:
+            level = i // 2
+        else:
+            level = net.levels - 2 - (i - 2 * net.levels) // 2
+        cin = IN if i == 0 else L["c1"] + L["c2"]
+        total += 2 * 9 * cin * L["cout"] * n * (h >> level) * (w >> level)
+    return total + 2 * net.base * OUT * n * h * w
+
+
+def time_network(net, n, h, w, reps=10):
+    """Forward and training-step times on random inputs."""
+    torch = _torch()
+    x = torch.rand((n, h, w, IN), dtype=torch.float32, device="cuda")
+    x[..., MASK] = 1.0
+    y = torch.rand((n, h, w, OUT), dtype=torch.float32, device="cuda")
+    net.forward(x)
+    fw = [net.forward(x)[1] for _ in range(reps)]
+    net.backward(x, y)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        net.backward(x, y)
+        net.adam(1e-5)
+    step_ms = (time.perf_counter() - t) / reps * 1e3
+    fl = conv_flops(net, n, h, w)
+    fms = float(np.median(fw))
+    return {"n": n, "h": h, "w": w, "forward_ms": fms, "forward_tflops": fl / (fms * 1e-3) / 1e12,
+            "train_step_ms": step_ms, "train_tflops": 3 * fl / (step_ms * 1e-3) / 1e12, "forward_flops": fl}
+
+
 def _seed_range(s):
     a, b = s.split(":")
     return list(range(int(a), int(b)))
@@ -355,8 +402,14 @@ def main(argv=None):
     t.add_argument("--windows", type=int, default=2)
     t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--batch", type=int, default=16)
+    t.add_argument("--crop", type=int, default=128)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
     t.add_argument("--out", default=WEIGHTS)
+    tm = sub.add_parser("time")
+    tm.add_argument("--weights", default=None)
+    tm.add_argument("--shape", default="1x2048x2048", help="n x h x w")
+    tm.add_argument("--reps", type=int, default=10)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="2000:2020")
@@ -368,13 +421,19 @@ def main(argv=None):
         t0 = time.perf_counter()
         data = make_dataset(_seed_range(args.seeds), args.windows, log=log)
         t1 = time.perf_counter()
-        net = Denoiser(4, 32, seed=1)
-        hist = train(net, data, args.steps, args.batch, lr=args.lr, log=log)
+        net = Denoiser.load(args.init) if args.init else Denoiser(4, 32, seed=1)
+        hist = train(net, data, args.steps, args.batch, args.crop, lr=args.lr, log=log)
         t2 = time.perf_counter()
-        meta = {"seeds": args.seeds, "windows": args.windows, "steps": args.steps, "batch": args.batch,
+        meta = {"seeds": args.seeds, "windows": args.windows, "init": args.init, "steps": args.steps, "batch": args.batch,
+                "crop": args.crop,
                 "lr": args.lr, "data_s": t1 - t0, "train_s": t2 - t1, "final_loss": float(np.mean(hist[-100:]))}
         net.save(args.out, meta)
         print(json.dumps(meta), flush=True)
+        return 0
+    if args.cmd == "time":
+        net = Denoiser.load(args.weights) if args.weights else Denoiser(4, 32, seed=1)
+        n, h, w = (int(v) for v in args.shape.split("x"))
+        print(json.dumps(time_network(net, n, h, w, args.reps)), flush=True)
         return 0
     torch = _torch()
     net = Denoiser.load(args.weights)

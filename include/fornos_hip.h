@@ -208,7 +208,16 @@ fornos_status fornos_solver_last_timing(const fornos_solver *s, float *rays_ms, 
 /* Traversal work of the last run_all of a hemisphere solver, counted by an
  * instrumented run of the same kernel: node tests, triangle tests, rays. */
 fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
-                                       uint64_t *tri_tests, uint64_t *rays);
+                                       uint64_t *tri_tests, uint64_t *rays, uint64_t *stack_overflows);
+/* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
+ * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
+ * stack entries per lane (16, 24 or 32; a ray that would overflow it is
+ * redone on a stackless tree, so results do not depend on it).  No reference
+ * counterpart (the shaders' dispatch shape is fixed). */
+fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu, uint32_t refill,
+                                       uint32_t stack);
+fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
+                                       uint32_t *stack);
 void fornos_solver_destroy(fornos_solver *s);
 
 /* ---- batched jobs -------------------------------------------------------- */
@@ -222,14 +231,15 @@ void fornos_solver_destroy(fornos_solver *s);
  * be idle when fornos_batch_run is called. */
 typedef struct fornos_batch fornos_batch;
 fornos_status fornos_batch_create(fornos_solver *const *solvers, uint32_t n, fornos_batch **out);
-/* Launch shape: persistent blocks of 256 lanes per CU (1..16) and the
- * number of idle lanes that triggers a wave's refill (1..64). */
-fornos_status fornos_batch_set_launch(fornos_batch *b, uint32_t blocks_per_cu, uint32_t refill);
-fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_per_cu, uint32_t *refill);
-/* Autotuner: times every (blocks_per_cu, refill) candidate on the first
- * `fraction` of each job's texels and keeps the fastest. */
+/* Launch shape, as fornos_solver_set_launch. */
+fornos_status fornos_batch_set_launch(fornos_batch *b, uint32_t blocks_per_cu, uint32_t refill, uint32_t stack);
+fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_per_cu, uint32_t *refill,
+                                      uint32_t *stack);
+/* Autotuner: times launch shapes on the first `fraction` of each job's
+ * texels by coordinate descent (stack depth {16,24,32}, then blocks per CU
+ * {4,6,8,12}, then refill {8,16,32}) and keeps the fastest. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
-                                    uint32_t *refill, float *best_ms);
+                                    uint32_t *refill, uint32_t *stack, float *best_ms);
 /* Runs every job once (graph launch); *ms = wall time of the graph on the
  * device (HIP events). */
 fornos_status fornos_batch_run(fornos_batch *b, float *ms);

@@ -39,6 +39,7 @@ def test_batch_arguments_are_checked():
     arr = (C.c_void_p * 1)(None)
     assert L.fornos_batch_create(arr, 1, C.byref(out)) == _abi.UNSUPPORTED
     assert b"AO solver" in L.fornos_last_error()
-    assert L.fornos_batch_set_launch(None, 4, 16) == _abi.INVALID_ARG
+    assert L.fornos_batch_set_launch(None, 4, 16, 24) == _abi.INVALID_ARG
     assert L.fornos_batch_run(None, None) == _abi.INVALID_ARG
-    assert L.fornos_batch_autotune(None, C.c_float(0.1), None, None, None) == _abi.INVALID_ARG
+    assert L.fornos_batch_autotune(None, C.c_float(0.1), None, None, None, None) == _abi.INVALID_ARG
+    assert L.fornos_solver_set_launch(None, 8, 16, 24) == _abi.INVALID_ARG

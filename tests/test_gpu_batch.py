@@ -41,8 +41,8 @@ def test_batch_after_autotune_and_launch_changes(jobs, tmp_path):
         seq.append(j.solver.getResults().view(np.uint32).copy())
     b = batch.Batch([j.solver for j in jobs])
     t = batch.tune(b, jobs, cache_path=str(tmp_path / "tune.json"), fraction=0.1)
-    assert not t["cached"] and b.get_launch() == (t["blocks_per_cu"], t["refill"])
-    for shape in [None, (1, 1), (3, 64), (16, 8)]:
+    assert not t["cached"] and b.get_launch() == (t["blocks_per_cu"], t["refill"], t["stack"])
+    for shape in [None, (1, 1, 16), (3, 64, 32), (16, 8, 24), (2, 5, 16)]:
         if shape:
             b.set_launch(*shape)
         b.run()
