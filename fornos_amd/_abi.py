@@ -104,6 +104,7 @@ _SIGS = [
     ("fornos_solver_last_timing", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_float), u32p]),
     ("fornos_solver_count_work", None, [H, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
+    ("fornos_solver_work_counters", None, [H, P, C.c_uint32]),
     ("fornos_solver_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("fornos_solver_get_launch", None, [H, u32p, u32p, u32p]),
     ("fornos_solver_destroy", "void", [H]),

@@ -335,9 +335,14 @@ class Solver(_Handle):
     def count_work(self):
         a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
-        return {"node_tests": a.value, "tri_tests": b.value, "rays": c.value, "stack_overflows": d.value}
+        out = np.zeros(10, np.uint64)
+        check(lib().fornos_solver_work_counters(self._h, ptr(out), 10))
+        w = {"node_tests": a.value, "tri_tests": b.value, "rays": c.value, "stack_overflows": d.value}
+        for k, v in zip(["walk_steps", "walk_lanes", "leaf_steps", "leaf_lanes", "refills", "refill_lanes"], out[4:]):
+            w[k] = int(v)
+        return w
 
-    def set_launch(self, blocks_per_cu=8, refill=16, stack=24):
+    def set_launch(self, blocks_per_cu=8, refill=16, stack=16):
         check(lib().fornos_solver_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
 
     def get_launch(self):
@@ -401,7 +406,7 @@ class Batch(_Handle):
         arr = (C.c_void_p * len(self.solvers))(*[s.handle.value for s in self.solvers])
         check(lib().fornos_batch_create(arr, len(self.solvers), C.byref(self._h)))
 
-    def set_launch(self, blocks_per_cu, refill, stack=24):
+    def set_launch(self, blocks_per_cu, refill, stack=16):
         check(lib().fornos_batch_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
 
     def get_launch(self):

@@ -66,6 +66,7 @@ constexpr float kSlowDir = 9.094947e-13f;  // 2^-40: smaller direction component
 
 struct Ray {
     V3 o, d, inv;
+    V3 oi;  // RN(o * inv): the culling test's slab planes are fma(m, inv, -oi)
     bool slow;
 };
 
@@ -95,29 +96,31 @@ __device__ inline bool node_visit(const Ray& r, float4 lo, float4 hi, float T)
 }
 
 // Conservative form of the same test for the culling tree: false only when
-// the box is missed for certain (its boxes are padded, so certainty suffices).
+// the box is missed for certain.  The culling boxes contain the reference
+// leaf boxes grown by pad = 2^-16 * max(1, max |coordinate|), which moves each
+// slab plane of a reachable leaf by pad * |1/d_i| in t.  The slab distances
+// here are fma(m, 1/d_i, -RN(o_i/d_i)) with |m|, |o_i| <= max |coordinate|
+// (+pad): their error against (m - o_i)/d_i is a few units in the last place
+// of max(|m|, |o_i|) * |1/d_i|, i.e. below 2^-21 * max(1, max |coordinate|) *
+// |1/d_i|, far inside the padding; the reference's own rounding of its
+// quotients is of the same order.  So a culling box is never rejected when a
+// reference leaf inside it passes the reference test, and no margin is
+// needed.  `al` (entry) orders children and prunes thickness entries.
 __device__ inline bool box_maybe(const Ray& r, float4 lo, float4 hi, float T, float& al)
 {
-    const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
-    const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
-    const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
+    const float t1x = __builtin_fmaf(lo.x, r.inv.x, -r.oi.x), t2x = __builtin_fmaf(hi.x, r.inv.x, -r.oi.x);
+    const float t1y = __builtin_fmaf(lo.y, r.inv.y, -r.oi.y), t2y = __builtin_fmaf(hi.y, r.inv.y, -r.oi.y);
+    const float t1z = __builtin_fmaf(lo.z, r.inv.z, -r.oi.z), t2z = __builtin_fmaf(hi.z, r.inv.z, -r.oi.z);
     const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
     const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
-    al = a - __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
-    const float bh = b + __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
-    return !(bh < 0.0f || al > bh || al >= T);
+    al = a;
+    return a <= b && b >= 0.0f && a < T;
 }
 
 __device__ inline bool node_maybe(const Ray& r, float4 lo, float4 hi, float T)
 {
-    const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
-    const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
-    const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
-    const float a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
-    const float b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
-    const float al = a - __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
-    const float bh = b + __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
-    return !(bh < 0.0f || al > bh || al >= T);
+    float al;
+    return box_maybe(r, lo, hi, T, al);
 }
 
 // AO and BN decide a ray on its first qualifying hit (ao_step1.comp:158,

@@ -31,19 +31,22 @@ namespace {
 
 enum { kAO = 0, kBN = 1, kThick = 2 };
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
+// counting run: box tests, triangle tests, rays, stack overflows, then per wave:
+// walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes
+constexpr uint32_t kStats = 10;
 constexpr uint32_t kRefill = 16;  // default: refill a wave once this many lanes are idle
-constexpr uint32_t kStack = 24;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
+constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
 
 struct RayArgs {
     DevScene sc;
     const float4* __restrict__ coords;
     const uint32_t* __restrict__ tidx;
-    const float4* __restrict__ samples;
+    const float4* __restrict__ samples;  // per permutation, in ray order; w = sample index
     uint32_t* __restrict__ cnt;         // AO: occluded rays per texel (global texel index)
     uint32_t* __restrict__ mask;        // BN: hit bits, mwords words per texel of the launch
     float* __restrict__ tval;           // thickness: closest t per ray of the launch
     unsigned int* __restrict__ counter;  // next ray of the launch to hand out
-    unsigned long long* __restrict__ stats;  // box tests, triangle tests, rays (COUNT only)
+    unsigned long long* __restrict__ stats;  // kStats counters (COUNT only)
     uint32_t S, P, t0, ntex, mwords;    // texels [t0, t0 + ntex), S rays each
     float mind, maxd;
     uint32_t refill;                    // refill a wave once this many lanes are idle
@@ -53,6 +56,7 @@ struct Lane {
     Ray ray;
     uint32_t q, s;      // texel of the launch, sample
     uint32_t cur, sp;   // 4-wide tree: next item (node, kLeafFlag | leaf, kNoChild = pop)
+    uint32_t slot;      // 4-wide tree: a found leaf waiting for the wave's leaf step (or kNoChild)
     uint32_t node;      // binary trees: next node
     float best;         // thickness: closest accepted hit
     int mode;
@@ -109,6 +113,8 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
     const uint32_t tid = threadIdx.x;
     const float T = KIND == kThick ? fminf(L.best, A.maxd) : A.maxd;
     if (L.cur == kNoChild) {
+        // pop, and visit the popped node in the same step (a popped leaf
+        // ends the walk; a dead thickness entry takes another step)
         if (L.sp == 0) {
             finish<KIND>(A, L, false);
             return;
@@ -117,7 +123,7 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
         L.cur = stk_node[L.sp * kBlock + tid];
         // thickness: a stacked subtree entered beyond the best hit is dead
         if (KIND == kThick && stk_t[L.sp * kBlock + tid] >= T) L.cur = kNoChild;
-        return;
+        if (L.cur == kNoChild || (L.cur & kLeafFlag)) return;
     }
     const float4* nd = A.sc.wide + 8 * (size_t)L.cur;
     const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
@@ -151,6 +157,7 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
         if (c[k] == kNoChild) continue;
         if (L.sp == STK) {  // redo the ray on the stackless binary culling tree
             if (COUNT) ++nov;
+            L.slot = kNoChild;
             L.mode = kCullTree;
             L.node = 0;
             L.best = FLT_MAX;
@@ -226,20 +233,26 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
     uint32_t total = A.ntex * A.S;
     const uint32_t lane = threadIdx.x & 63;
     Lane L;
-    L.ray.o = L.ray.d = L.ray.inv = mk(0, 0, 0);
+    L.ray.o = L.ray.d = L.ray.inv = L.ray.oi = mk(0, 0, 0);
     L.ray.slow = false;
     L.q = L.s = L.cur = L.sp = L.node = 0;
+    L.slot = kNoChild;
     L.best = FLT_MAX;
     L.mode = kWideTree;
     L.active = false;
     bool exhausted = false;
     uint32_t cn = 0, ct = 0, nr = 0, nov = 0;
+    unsigned long long w_steps = 0, w_lanes = 0, l_steps = 0, l_lanes = 0, r_steps = 0, r_lanes = 0;
     for (;;) {
         // refill idle lanes, one atomic per wave
         const uint64_t idle = __ballot(!L.active);
         const uint32_t nidle = __popcll(idle);
         if (!exhausted && (nidle >= A.refill || nidle == 64)) {
             uint32_t base = 0;
+            if (COUNT && lane == 0) {
+                ++r_steps;
+                r_lanes += nidle;
+            }
             if (lane == 0) base = atomicAdd(A.counter, nidle);
             base = __shfl(base, 0);
             if (base + nidle >= total) exhausted = true;
@@ -252,15 +265,18 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                         const Basis B = ray_basis(A.sc, A.coords[g], t);
                         const V3 d = KIND == kThick ? -B.d : B.d;
                         L.ray.o = B.o;
-                        L.ray.d = sample_dir(B, d, A.samples[(size_t)(g % A.P) * A.S + s]);
+                        const float4 rs = A.samples[(size_t)(g % A.P) * A.S + s];
+                        L.ray.d = sample_dir(B, d, rs);
                         L.ray.inv = mk(1.0f / L.ray.d.x, 1.0f / L.ray.d.y, 1.0f / L.ray.d.z);
+                        L.ray.oi = mk(L.ray.o.x * L.ray.inv.x, L.ray.o.y * L.ray.inv.y, L.ray.o.z * L.ray.inv.z);
                         L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
                                        fabsf(L.ray.d.z) >= kSlowDir);
                         L.mode = L.ray.slow ? kRefTree : kWideTree;
                         L.q = q;
-                        L.s = s;
+                        L.s = __float_as_uint(rs.w);  // the sample's index in the table
                         L.cur = 0;
                         L.sp = 0;
+                        L.slot = kNoChild;
                         L.node = 0;
                         L.best = FLT_MAX;
                         L.active = true;
@@ -278,18 +294,43 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
             exhausted = false;
             continue;
         }
-        // phase 1: walk the 4-wide tree until every lane holds a leaf or is done
+        // phase 1: walk the 4-wide tree until every lane has parked a leaf (or
+        // is done).  Speculative: a lane that has parked one keeps walking
+        // while others search, and stops at its next leaf.
         for (;;) {
-            const bool walk = L.active && L.mode == kWideTree && (L.cur == kNoChild || !(L.cur & kLeafFlag));
-            if (__ballot(walk) == 0) break;
+            const bool wide = L.active && L.mode == kWideTree;
+            const bool at_leaf = L.cur != kNoChild && (L.cur & kLeafFlag);
+            if (wide && at_leaf && L.slot == kNoChild) {
+                L.slot = L.cur;
+                L.cur = kNoChild;
+            }
+            if (__ballot(wide && L.slot == kNoChild) == 0) break;
+            // thickness does not walk past its leaf: its pruning needs the hits first
+            const bool walk = wide && (KIND != kThick || L.slot == kNoChild) &&
+                              !(L.cur != kNoChild && (L.cur & kLeafFlag)) &&
+                              !(L.cur == kNoChild && L.sp == 0 && L.slot != kNoChild);
+            if (COUNT) {
+                const uint64_t wb = __ballot(walk);
+                if (lane == 0) {
+                    ++w_steps;
+                    w_lanes += __popcll(wb);
+                }
+            }
             if (walk) wide_step<KIND, COUNT, STK>(A, L, stk_node, stk_t, cn, nov);
         }
-        // phase 2: leaves together; rays on a binary tree take one step
+        // phase 2: parked leaves together; rays on a binary tree take one step
+        if (COUNT) {
+            const uint64_t lb = __ballot(L.active);
+            if (lane == 0) {
+                ++l_steps;
+                l_lanes += __popcll(lb);
+            }
+        }
         if (L.active) {
             if (L.mode == kWideTree) {
-                const uint4 lf = A.sc.wleaf[L.cur & ~kLeafFlag];
+                const uint4 lf = A.sc.wleaf[L.slot & ~kLeafFlag];
+                L.slot = kNoChild;
                 const int r = leaf_hits<KIND, COUNT>(A, L, lf.x, lf.y, lf.z, ct);
-                L.cur = kNoChild;
                 if (r == 1) {
                     finish<KIND>(A, L, true);
                 } else if (r == 2) {
@@ -311,6 +352,12 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
             o += __shfl_down(o, off);
         }
         if (lane == 0) {
+            atomicAdd(&A.stats[4], w_steps);
+            atomicAdd(&A.stats[5], w_lanes);
+            atomicAdd(&A.stats[6], l_steps);
+            atomicAdd(&A.stats[7], l_lanes);
+            atomicAdd(&A.stats[8], r_steps);
+            atomicAdd(&A.stats[9], r_lanes);
             atomicAdd(&A.stats[3], o);
             atomicAdd(&A.stats[0], a);
             atomicAdd(&A.stats[1], b);

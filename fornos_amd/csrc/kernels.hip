@@ -74,10 +74,11 @@ struct fornos_solver {
     fornos_meshmap* mm = nullptr;
     uint32_t P = 0;
     uint64_t work_total = 0, offset = 0;  // work items (rays or texels)
-    float4* samples = nullptr;
+    float4* samples = nullptr;      // sample table, reference order (reductions)
+    float4* ray_samples = nullptr;  // the same samples sorted by direction, w = sample index
     float* out = nullptr;
     uint32_t* counts = nullptr;
-    unsigned long long* stats = nullptr;
+    unsigned long long* stats = nullptr;  // kStats counters of the counting run
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
@@ -90,7 +91,7 @@ struct fornos_solver {
     ~fornos_solver()
     {
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
-        dev_free(samples); dev_free(out); dev_free(counts); dev_free(stats);
+        dev_free(samples); dev_free(ray_samples); dev_free(out); dev_free(counts); dev_free(stats);
         dev_free(raybuf); dev_free(counter);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
@@ -132,7 +133,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.sc = sc->d;
     A.coords = mm->coords;
     A.tidx = mm->tidx;
-    A.samples = s->samples;
+    A.samples = s->ray_samples;
     const uint32_t mwords = (S + 31) / 32;
     A.cnt = s->counts;
     A.mask = KIND == kBN ? (uint32_t*)s->raybuf : nullptr;
@@ -163,7 +164,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     auto fill = [&](void* p, size_t n, uint32_t v) {
         hipLaunchKernelGGL(k_fill, dim3(div_up(n, kBlock)), dim3(kBlock), 0, sc->stream, (uint32_t*)p, n, v);
     };
-    if (count) fill(s->stats, 6, 0);
+    if (count) fill(s->stats, 2 * kStats, 0);
     for (uint32_t c0 = t0; c0 < t1; c0 += s->chunk_texels) {
         const uint32_t nt = std::min(s->chunk_texels, t1 - c0);
         A.t0 = R.t0 = c0;
@@ -621,8 +622,42 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         for (size_t i = 0; i < t4.size(); ++i) t4[i] = make_float4(table[3 * i], table[3 * i + 1], table[3 * i + 2], 0.0f);
         if (fornos_status st = dev_alloc(&s->samples, t4.size())) return st;
         HIP_TRY(hipMemcpy(s->samples, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));
+        // ray order: each permutation's samples sorted by the Morton code of
+        // their disk position (x, y), so that consecutive rays of a texel --
+        // the lanes of a wave -- point in similar directions.  w carries the
+        // sample's index, under which its result is stored.
+        {
+            std::vector<float4> r4(t4.size());
+            std::vector<std::pair<uint32_t, uint32_t>> key(S);
+            auto spread = [](uint32_t v) {  // 16 bits -> even bit positions
+                v &= 0xffffu;
+                v = (v | (v << 8)) & 0x00ff00ffu;
+                v = (v | (v << 4)) & 0x0f0f0f0fu;
+                v = (v | (v << 2)) & 0x33333333u;
+                v = (v | (v << 1)) & 0x55555555u;
+                return v;
+            };
+            auto quant = [](float x) {
+                const float u = std::min(1.0f, std::max(0.0f, 0.5f * (x + 1.0f)));
+                return (uint32_t)(u * 65535.0f);
+            };
+            for (uint32_t p = 0; p < n_perm; ++p) {
+                const float4* row = &t4[(size_t)p * S];
+                for (uint32_t i = 0; i < S; ++i) key[i] = {spread(quant(row[i].x)) | (spread(quant(row[i].y)) << 1), i};
+                std::stable_sort(key.begin(), key.end(),
+                                 [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                                     return a.first < b.first;
+                                 });
+                for (uint32_t j = 0; j < S; ++j) {
+                    const float4 v = row[key[j].second];
+                    r4[(size_t)p * S + j] = make_float4(v.x, v.y, v.z, __builtin_bit_cast(float, key[j].second));
+                }
+            }
+            if (fornos_status st = dev_alloc(&s->ray_samples, r4.size())) return st;
+            HIP_TRY(hipMemcpy(s->ray_samples, r4.data(), sizeof(float4) * r4.size(), hipMemcpyHostToDevice));
+        }
         if (fornos_status st = dev_alloc(&s->counts, mm->n)) return st;
-        if (fornos_status st = dev_alloc(&s->stats, 3)) return st;
+        if (fornos_status st = dev_alloc(&s->stats, kStats)) return st;
         if (fornos_status st = dev_alloc(&s->counter, 1)) return st;
         // per-ray results of one k_rays launch of at most 2^28 rays: hit bits
         // (BN, 32 MiB) or t values (thickness, 1 GiB); AO counts in place
@@ -742,13 +777,23 @@ fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64
     if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: null handle");
     if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: not a hemisphere solver");
     if (fornos_status st = solver_launch(s, 0, s->mm->n, true)) return st;
-    unsigned long long h[4] = {0, 0, 0, 0};
+    unsigned long long h[kStats] = {};
     HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
     HIP_TRY(hipMemcpy(h, s->stats, sizeof(h), hipMemcpyDeviceToHost));
     if (nodes) *nodes = h[0];
     if (tris) *tris = h[1];
     if (rays) *rays = h[2];
     if (overflows) *overflows = h[3];
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_work_counters(const fornos_solver* s, uint64_t* counters, uint32_t n)
+{
+    if (!s || !counters || n == 0 || n > kStats)
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_work_counters: 1 to " + std::to_string(kStats) + " counters");
+    if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_work_counters: not a hemisphere solver");
+    HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
+    HIP_TRY(hipMemcpy(counters, s->stats, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
     return FORNOS_OK;
 }
 
@@ -863,7 +908,7 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         A.sc = s->mm->sc->d;
         A.coords = s->mm->coords;
         A.tidx = s->mm->tidx;
-        A.samples = s->samples;
+        A.samples = s->ray_samples;
         A.cnt = s->counts;
         A.counter = b->counters + j;
         A.stats = s->stats;

@@ -209,6 +209,11 @@ fornos_status fornos_solver_last_timing(const fornos_solver *s, float *rays_ms, 
  * instrumented run of the same kernel: node tests, triangle tests, rays. */
 fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
                                        uint64_t *tri_tests, uint64_t *rays, uint64_t *stack_overflows);
+/* All counters of the last fornos_solver_count_work run: box tests, triangle
+ * tests, rays, stack overflows; then, summed over waves: walk steps, lanes
+ * walking, leaf steps, lanes in leaf steps, refills, lanes refilled (their
+ * ratios are the SIMD lane utilisation of each phase).  n <= 10. */
+fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *counters, uint32_t n);
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
  * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
  * stack entries per lane (16, 24 or 32; a ray that would overflow it is

@@ -189,3 +189,30 @@ def test_ao_sample_counts(field, S):
     out, oc = orc.hemisphere(orc.KIND_AO, S, 0.01, 0.2, s.table, field.coords, field.tidx, field.spos,
                              field.snrm, field.nodes, 0, 128)
     assert np.array_equal(cnt[:128], oc) and same_bits(res[:128], out)
+
+
+@pytest.mark.parametrize("kind", ["ao", "bentnormals", "thickness"])
+def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
+    # the stack depth only decides which rays are redone on the stackless
+    # tree; the launch shape only how rays are handed out
+    p = field.cfg["maps"][kind]
+    cls = {"ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
+           "thickness": fa.ThicknessSolver}[kind]
+    s = cls(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
+    assert s.get_launch() == (8, 16, 16)
+    s.run_all()
+    ref = s.getResults().view(np.uint32).copy()
+    live = int(np.sum(field.tidx != MISS))
+    for shape in [(8, 16, 24), (8, 16, 32), (1, 1, 16), (3, 64, 24), (16, 7, 32)]:
+        s.set_launch(*shape)
+        assert s.get_launch() == shape
+        s.reset()
+        s.run_all()
+        assert np.array_equal(s.getResults().view(np.uint32), ref), shape
+        w = s.count_work()
+        assert w["rays"] == live * p["sampleCount"]
+        assert w["node_tests"] > w["rays"] and w["tri_tests"] > 0
+        assert 0 <= w["stack_overflows"] <= w["rays"]
+        assert np.array_equal(s.getResults().view(np.uint32), ref), shape
+    with pytest.raises(fa.FornosError):
+        s.set_launch(8, 16, 20)
