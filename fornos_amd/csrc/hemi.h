@@ -35,6 +35,7 @@ enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ra
 // walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes
 constexpr uint32_t kStats = 10;
 constexpr uint32_t kRefill = 16;  // default: refill a wave once this many lanes are idle
+constexpr uint32_t kQuorum = 8;   // leaf step once searching lanes <= parked lanes / kQuorum (0: wait for all)
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
 
 struct RayArgs {
@@ -304,7 +305,14 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                 L.slot = L.cur;
                 L.cur = kNoChild;
             }
-            if (__ballot(wide && L.slot == kNoChild) == 0) break;
+            const uint64_t need = __ballot(wide && L.slot == kNoChild);
+            if (need == 0) break;
+            if (kQuorum) {
+                // leaf step once few lanes are still searching (they go on
+                // walking after it); never without a parked leaf
+                const uint64_t parked = __ballot(wide && L.slot != kNoChild);
+                if (parked != 0 && __popcll(need) * kQuorum <= (uint32_t)__popcll(parked)) break;
+            }
             // thickness does not walk past its leaf: its pruning needs the hits first
             const bool walk = wide && (KIND != kThick || L.slot == kNoChild) &&
                               !(L.cur != kNoChild && (L.cur & kLeafFlag)) &&
@@ -327,7 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
             }
         }
         if (L.active) {
-            if (L.mode == kWideTree) {
+            if (L.mode != kWideTree) {
+                binary_step<KIND, COUNT>(A, L, cn, ct);
+            } else if (L.slot != kNoChild) {  // (a lane still searching walks on)
                 const uint4 lf = A.sc.wleaf[L.slot & ~kLeafFlag];
                 L.slot = kNoChild;
                 const int r = leaf_hits<KIND, COUNT>(A, L, lf.x, lf.y, lf.z, ct);
@@ -338,8 +348,6 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                     L.node = 0;
                     L.best = FLT_MAX;
                 }
-            } else {
-                binary_step<KIND, COUNT>(A, L, cn, ct);
             }
         }
     }
