@@ -140,14 +140,12 @@ __device__ inline bool chain_ok(const Ray& r, const DevScene& sc, uint32_t leaf,
 // Shaders/ao_step1.comp:109-147 with the per-triangle terms read from `tri`.
 // Returns t, or FLT_MAX.  Triangles whose t falls outside [tmin, tmax) cannot
 // change the caller's result and are rejected before the barycentrics.
-__device__ inline float hemi_tri_q(V3 o, V3 d, float4 q0, uint32_t k, const DevScene& sc, float tmin,
+// Second stage of the triangle test: t = num / nd (num = a.n - o.n, nd =
+// d.n, both as the shader computes them), its range, then the barycentrics.
+__device__ inline float hemi_tri_t(V3 o, V3 d, float num, float nd, uint32_t k, const DevScene& sc, float tmin,
                                    float tmax)
 {
-    const V3 n = xyz(q0);
-    const float nd = dot(d, n);
-    if (!(fabsf(nd) > 0)) return FLT_MAX;
-    const float pn = dot(o, n);
-    const float t = (q0.w - pn) / nd;
+    const float t = num / nd;
     if (!(t >= 0 && t >= tmin && t < tmax)) return FLT_MAX;
     const float4 q1 = sc.trest[3 * k], q2 = sc.trest[3 * k + 1], q3 = sc.trest[3 * k + 2];
     const V3 p = o + d * t;
@@ -159,6 +157,16 @@ __device__ inline float hemi_tri_q(V3 o, V3 d, float4 q0, uint32_t k, const DevS
     const float z = (d00 * d21 - d01 * d20) / denom;
     const float x = 1.0f - y - z;
     return (x >= 0 && y >= 0 && y <= 1 && z >= 0 && z <= 1) ? t : FLT_MAX;
+}
+
+__device__ inline float hemi_tri_q(V3 o, V3 d, float4 q0, uint32_t k, const DevScene& sc, float tmin,
+                                   float tmax)
+{
+    const V3 n = xyz(q0);
+    const float nd = dot(d, n);
+    if (!(fabsf(nd) > 0)) return FLT_MAX;
+    const float pn = dot(o, n);
+    return hemi_tri_t(o, d, q0.w - pn, nd, k, sc, tmin, tmax);
 }
 
 __device__ inline float hemi_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmin, float tmax)
