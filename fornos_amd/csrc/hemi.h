@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                 r_lanes += nidle;
             }
             if (lane == 0) base = atomicAdd(A.counter, nidle);
-            base = __shfl(base, 0);
+            base = __builtin_amdgcn_readfirstlane(base);  // wave-uniform, so the job state stays scalar
             if (base + nidle >= total) exhausted = true;
             if (!L.active) {
                 const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));

@@ -980,9 +980,13 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
 {
     if (!b || !(fraction > 0.0f) || fraction > 1.0f)
         return set_error(FORNOS_INVALID_ARG, "fornos_batch_autotune: fraction must be in (0, 1]");
-    // a prefix of each job's texels (row-major: a band of the map)
+    // a prefix of each job's texels (row-major: a band of the map): the
+    // fraction, but at least 2^22 rays, so each job reaches its steady state
     std::vector<RayArgs> sample = b->jobs;
-    for (RayArgs& A : sample) A.ntex = std::max<uint32_t>(std::min<uint32_t>(A.ntex, 1), (uint32_t)(A.ntex * fraction));
+    for (RayArgs& A : sample) {
+        const uint32_t floor_tex = std::min<uint32_t>(A.ntex, div_up(1u << 22, A.S));
+        A.ntex = std::max<uint32_t>(floor_tex, (uint32_t)(A.ntex * fraction));
+    }
     RayArgs* dsample = nullptr;
     if (fornos_status st = dev_alloc(&dsample, sample.size())) return st;
     struct Free { RayArgs* p; ~Free() { dev_free(p); } } guard{dsample};

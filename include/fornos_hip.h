@@ -241,7 +241,7 @@ fornos_status fornos_batch_set_launch(fornos_batch *b, uint32_t blocks_per_cu, u
 fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_per_cu, uint32_t *refill,
                                       uint32_t *stack);
 /* Autotuner: times launch shapes on the first `fraction` of each job's
- * texels by coordinate descent (stack depth {16,24,32}, then blocks per CU
+ * texels (at least 2^22 rays of each job) by coordinate descent (stack depth {16,24,32}, then blocks per CU
  * {4,6,8,12}, then refill {8,16,32}) and keeps the fastest. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
                                     uint32_t *refill, uint32_t *stack, float *best_ms);
