@@ -184,18 +184,14 @@ def main():
 
     bake = None
     if not args.no_bake:
-        # seconds per full 2048^2 bake: mapping + every map the workload names
-        t0 = time.perf_counter()
-        mm2 = fa.MeshMapping().init(prep["texels"], prep["scene"], cfg["cull"])
-        parts = {"mapping": mm2.run_all()}
-        for name, kw in cfg["maps"].items():
-            cls = {"height": fa.HeightSolver, "position": fa.PositionSolver, "normals": fa.NormalsSolver,
-                   "ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
-                   "thickness": fa.ThicknessSolver}[name]
-            s = cls(**kw).init(prep["texels"], mm2)
-            parts[name] = s.run_all()
-            s.getResults()
-        bake = {"seconds": time.perf_counter() - t0, "kernel_ms": parts}
+        # seconds per full bake of the workload: mapping + every map it names,
+        # then the exports each reference task writes (8-bit PNG, dilation 16)
+        import tempfile
+        from fornos_amd import bake as fb
+        with tempfile.TemporaryDirectory() as td:
+            rep = fb.bake(cfg, td, "png", prep=prep)
+        bake = {"seconds": rep["bake_s"], "export_seconds": rep["export_s"],
+                "kernel_ms": {k[:-3]: v for k, v in rep.items() if k.endswith("_ms")}}
 
     denoise = None if args.no_denoise else denoise_leg(prep)
     cpu = None if args.no_cpu else cpu_baseline(prep, mm, ao)

@@ -130,6 +130,19 @@ _SIGS = [
     ("fornos_denoise_scatter", None, [H, P, P, P]),
     ("fornos_denoise_check_tiles", None, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
                                           C.c_float, P]),
+    # export (include/fornos_export.h)
+    ("fornos_export_float_rgb8", None, [H, C.c_float, C.c_float, C.c_int, C.c_int, P, P]),
+    ("fornos_export_float_f32", None, [H, C.c_float, C.c_float, C.c_int, P, P]),
+    ("fornos_export_normal_rgb8", None, [H, C.c_int, P]),
+    ("fornos_export_vector_f32", None, [H, P]),
+    ("fornos_solver_map_size", None, [H, u32p, u32p]),
+    ("fornos_export_float_image", None, [H, C.c_char_p, C.c_float, C.c_float, C.c_int, C.c_int, P]),
+    ("fornos_export_normal_image", None, [H, C.c_char_p, C.c_int]),
+    ("fornos_export_vector_image", None, [H, C.c_char_p]),
+    ("fornos_write_png", None, [C.c_char_p, C.c_uint32, C.c_uint32, P]),
+    ("fornos_write_tga", None, [C.c_char_p, C.c_uint32, C.c_uint32, P]),
+    ("fornos_write_exr", None, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
+    ("fornos_float_to_half", "void", [P, P, C.c_size_t]),
     ("fornos_dn_conv3x3", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P,
                                  C.c_uint32, C.c_int, P]),
     ("fornos_dn_conv3x3_wgrad", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,

@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 
-HOST_SOURCES = ["mesh.cpp", "texels.cpp", "bvh.cpp", "fornos.cpp"]
+HOST_SOURCES = ["mesh.cpp", "texels.cpp", "bvh.cpp", "fornos.cpp", "imageio.cpp"]
 HIP_SOURCES = ["kernels.hip", "denoise.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
@@ -55,7 +55,7 @@ def build(verbose=False):
                 print(o)
     objs = [os.path.join(OBJ, s + ".o") for s in HOST_SOURCES + HIP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

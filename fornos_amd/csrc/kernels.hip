@@ -22,8 +22,10 @@
 #include "mapping.h"
 #include "texel.h"
 #include "raster.h"
+#include "export.h"
 #include "hiphost.h"
 #include "../../include/fornos_denoise.h"
+#include "../../include/fornos_export.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -1199,6 +1201,202 @@ fornos_status fornos_denoise_check_tiles(const float* input, const float* output
                        slack, fail);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
+    return FORNOS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Image export on the device (include/fornos_export.h)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { dev_free(p); }
+};
+
+fornos_status export_check(const fornos_solver* s, uint32_t comps, const char* what)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, std::string(what) + ": null solver");
+    if (s->comps() != comps)
+        return set_error(FORNOS_INVALID_ARG, std::string(what) + ": needs a solver with " + std::to_string(comps) +
+                                                 " value(s) per texel");
+    if (!s->mm->indices)
+        return set_error(FORNOS_INVALID_ARG, std::string(what) + ": the texel set has no map indices");
+    return FORNOS_OK;
+}
+
+// getMinMax + the filter + computeScaleBias (Src/image.cpp:37-47, 72-80, 199-207)
+fornos_status export_range(const fornos_solver* s, float lo, float hi, int normalize, QuantArgs& A, float* minmax)
+{
+    const fornos_meshmap* mm = s->mm;
+    hipStream_t st = mm->sc->stream;
+    float tmin = FLT_MAX, tmax = -FLT_MAX;
+    if (mm->n) {
+        DevBuf<uint32_t> keys;
+        if (fornos_status e = dev_alloc(&keys.p, 2)) return e;
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys.p, (size_t)1, 0xffffffffu);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys.p + 1, (size_t)1, 0u);
+        hipLaunchKernelGGL(k_minmax, dim3(std::min<uint32_t>(1024, div_up(mm->n, kBlock))), dim3(kBlock), 0, st,
+                           (const float*)s->out, mm->n, keys.p);
+        uint32_t k[2];
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipMemcpy(k, keys.p, sizeof(k), hipMemcpyDeviceToHost));
+        auto inv = [](uint32_t key) {
+            const uint32_t u = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        };
+        if (k[0] != 0xffffffffu) tmin = inv(k[0]);
+        if (k[1] != 0u) tmax = inv(k[1]);
+    }
+    const bool filter = hi > lo;
+    const float mn = filter ? std::fmax(tmin, lo) : tmin, mx = filter ? std::fmin(tmax, hi) : tmax;
+    if (minmax) {
+        minmax[0] = mn;
+        minmax[1] = mx;
+    }
+    float scale = 1.0f, bias = 0.0f;
+    if (normalize && mn != FLT_MAX && mx != -FLT_MAX) {
+        scale = 1.0f / (mx - mn);
+        bias = -mn * scale;
+    }
+    A.v = s->out;
+    A.indices = mm->indices;
+    A.n = mm->n;
+    A.w = mm->width;
+    A.h = mm->height;
+    A.lo = lo;
+    A.hi = hi;
+    A.scale = scale;
+    A.bias = bias;
+    A.filter = filter ? 1 : 0;
+    A.normalize = normalize ? 1 : 0;
+    return FORNOS_OK;
+}
+
+// rgb (device, w*h*3) -> dilated copy back to the host
+fornos_status export_finish_rgb(const fornos_meshmap* mm, uint8_t* drgb, bool valid_from_rgb, int dilate,
+                                uint8_t* rgb)
+{
+    hipStream_t st = mm->sc->stream;
+    const size_t npix = (size_t)mm->width * mm->height;
+    DevBuf<uint8_t> valid, out;
+    const uint8_t* res = drgb;
+    if (dilate > 0 && npix) {
+        if (fornos_status e = dev_alloc(&valid.p, npix)) return e;
+        if (fornos_status e = dev_alloc(&out.p, npix * 3)) return e;
+        HIP_TRY(hipMemsetAsync(valid.p, 0, npix, st));
+        if (valid_from_rgb) {
+            if (mm->n)
+                hipLaunchKernelGGL(k_valid_rgb, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st,
+                                   (const uint32_t*)mm->indices, mm->n, mm->width, mm->height, (const uint8_t*)drgb,
+                                   valid.p);
+        } else if (mm->n) {
+            hipLaunchKernelGGL(k_valid_texels, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st,
+                               (const uint32_t*)mm->indices, mm->n, valid.p);
+        }
+        HIP_TRY(hipMemcpyAsync(out.p, drgb, npix * 3, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_dilate, dim3(div_up(mm->width, kBlock), mm->height), dim3(kBlock), 0, st,
+                           (const uint8_t*)drgb, out.p, (const uint8_t*)valid.p, mm->width, mm->height,
+                           (uint32_t)dilate);
+        res = out.p;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    if (npix) HIP_TRY(hipMemcpy(rgb, res, npix * 3, hipMemcpyDeviceToHost));
+    return FORNOS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fornos_status fornos_export_float_rgb8(const fornos_solver* s, float filter_lo, float filter_hi, int normalize,
+                                       int dilate, uint8_t* rgb, float* minmax)
+{
+    if (!rgb) return set_error(FORNOS_INVALID_ARG, "fornos_export_float_rgb8: null output");
+    if (fornos_status e = export_check(s, 1, "fornos_export_float_rgb8")) return e;
+    QuantArgs A{};
+    if (fornos_status e = export_range(s, filter_lo, filter_hi, normalize, A, minmax)) return e;
+    const fornos_meshmap* mm = s->mm;
+    const size_t npix = (size_t)mm->width * mm->height;
+    DevBuf<uint8_t> d;
+    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
+    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3, mm->sc->stream));
+    if (mm->n) hipLaunchKernelGGL(k_quant_gray, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
+    return export_finish_rgb(mm, d.p, false, dilate, rgb);
+}
+
+fornos_status fornos_export_float_f32(const fornos_solver* s, float filter_lo, float filter_hi, int normalize,
+                                      float* image, float* minmax)
+{
+    if (!image) return set_error(FORNOS_INVALID_ARG, "fornos_export_float_f32: null output");
+    if (fornos_status e = export_check(s, 1, "fornos_export_float_f32")) return e;
+    QuantArgs A{};
+    if (fornos_status e = export_range(s, filter_lo, filter_hi, normalize, A, minmax)) return e;
+    const fornos_meshmap* mm = s->mm;
+    const size_t npix = (size_t)mm->width * mm->height;
+    DevBuf<float> d;
+    if (fornos_status e = dev_alloc(&d.p, npix)) return e;
+    HIP_TRY(hipMemsetAsync(d.p, 0, npix * sizeof(float), mm->sc->stream));
+    if (mm->n) hipLaunchKernelGGL(k_float_plane, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
+    if (npix) HIP_TRY(hipMemcpy(image, d.p, npix * sizeof(float), hipMemcpyDeviceToHost));
+    return FORNOS_OK;
+}
+
+fornos_status fornos_export_normal_rgb8(const fornos_solver* s, int dilate, uint8_t* rgb)
+{
+    if (!rgb) return set_error(FORNOS_INVALID_ARG, "fornos_export_normal_rgb8: null output");
+    if (fornos_status e = export_check(s, 3, "fornos_export_normal_rgb8")) return e;
+    const fornos_meshmap* mm = s->mm;
+    QuantArgs A{};
+    A.v = s->out;
+    A.indices = mm->indices;
+    A.n = mm->n;
+    A.w = mm->width;
+    A.h = mm->height;
+    const size_t npix = (size_t)mm->width * mm->height;
+    DevBuf<uint8_t> d;
+    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
+    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3, mm->sc->stream));
+    if (mm->n) hipLaunchKernelGGL(k_quant_normal, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
+    return export_finish_rgb(mm, d.p, true, dilate, rgb);
+}
+
+fornos_status fornos_export_vector_f32(const fornos_solver* s, float* planes)
+{
+    if (!planes) return set_error(FORNOS_INVALID_ARG, "fornos_export_vector_f32: null output");
+    if (fornos_status e = export_check(s, 3, "fornos_export_vector_f32")) return e;
+    const fornos_meshmap* mm = s->mm;
+    QuantArgs A{};
+    A.v = s->out;
+    A.indices = mm->indices;
+    A.n = mm->n;
+    A.w = mm->width;
+    A.h = mm->height;
+    const size_t npix = (size_t)mm->width * mm->height;
+    DevBuf<float> d;
+    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
+    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3 * sizeof(float), mm->sc->stream));
+    if (mm->n) hipLaunchKernelGGL(k_vector_planes, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
+    if (npix) HIP_TRY(hipMemcpy(planes, d.p, npix * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_map_size(const fornos_solver* s, uint32_t* width, uint32_t* height)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_map_size: null solver");
+    if (width) *width = s->mm->width;
+    if (height) *height = s->mm->height;
     return FORNOS_OK;
 }
 
