@@ -70,8 +70,11 @@ _SIGS = [
     ("fornos_mesh_compute_vertex_normals", None, [H]),
     ("fornos_mesh_compute_vertex_normals_aggressive", None, [H]),
     ("fornos_mesh_compute_tangent_space", None, [H]),
+    ("fornos_mesh_compute_face_normals", None, [H]),
+    ("fornos_mesh_load", None, [C.c_char_p, C.POINTER(H)]),
     ("fornos_mesh_destroy", "void", [H]),
     ("fornos_texels_create", None, [H, H, C.c_uint32, C.c_uint32, C.POINTER(H)]),
+    ("fornos_texels_create_hybrid", None, [H, H, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(H)]),
     ("fornos_texels_view_get", None, [H, C.POINTER(TexelView)]),
     ("fornos_texels_export_mapuv", None, [H, P, P, P, P, P]),
     ("fornos_texels_destroy", "void", [H]),

@@ -49,6 +49,13 @@ class Mesh(_Handle):
         return m
 
     @classmethod
+    def load(cls, path):
+        """Mesh::loadFile (Src/mesh.cpp:612-617): .obj or .ply."""
+        m = cls()
+        check(lib().fornos_mesh_load(str(path).encode(), C.byref(m._h)))
+        return m
+
+    @classmethod
     def from_arrays(cls, positions, vertices, triangles, normals=None, texcoords=None):
         m = cls()
         keep = [np.ascontiguousarray(x, dtype=t) if x is not None else None for x, t in
@@ -89,6 +96,9 @@ class Mesh(_Handle):
     def n_triangles(self):
         return self.mesh_view().n_triangles
 
+    def computeFaceNormals(self):
+        check(lib().fornos_mesh_compute_face_normals(self._h))
+
     def computeVertexNormals(self):
         check(lib().fornos_mesh_compute_vertex_normals(self._h))
 
@@ -103,10 +113,11 @@ class CompressedMapUV(_Handle):
     """MapUV + CompressedMapUV (Src/compute.h:214-259) of a low-poly mesh."""
     _destroy = "fornos_texels_destroy"
 
-    def __init__(self, lo, dirs, width, height):
+    def __init__(self, lo, dirs, width, height, edge=0.0):
+        """edge > 0: the Hybrid mapping method (MapUV::fromMeshes_Hybrid)."""
         super().__init__()
-        check(lib().fornos_texels_create(lo.handle, dirs.handle if dirs is not None else None,
-                                         int(width), int(height), C.byref(self._h)))
+        check(lib().fornos_texels_create_hybrid(lo.handle, dirs.handle if dirs is not None else None,
+                                                int(width), int(height), float(edge), C.byref(self._h)))
         self.width, self.height = int(width), int(height)
 
     def texel_view(self):

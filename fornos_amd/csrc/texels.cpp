@@ -41,9 +41,24 @@ V3 blend(V3 a, V3 b, V3 c, V3 w) { return a * w.x + b * w.y + c * w.z; }
 
 }  // namespace
 
-bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
-                  fornos_texels& out)
+// Src/compute.cpp:195-210
+float edge_distance(V3 e0, V3 e1, V3 p)
 {
+    const V3 v = p - e0, e = e1 - e0;
+    const float t = dot(v, e) / dot(e, e);
+    const V3 w = (e0 + e * t) - p;
+    return std::sqrt(dot(w, w));
+}
+
+// Src/compute.cpp:75-193 (edge <= 0: createMapUV, the Smooth and
+// LowPolyNormals methods) or Src/compute.cpp:213-345 (edge > 0:
+// createMapUVEdge, the Hybrid method: bounds clamped at 0 only, tangents
+// indexed by normal index, direction blended toward the surface normal by
+// min(triangleDistance / edge, 1)).
+bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                  fornos_texels& out, float edge)
+{
+    const bool hybrid = edge > 0.0f;
     const size_t npx = (size_t)W * H;
     out.width = W;
     out.height = H;
@@ -69,14 +84,24 @@ bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, ui
             u[k] = lo.texcoords[lo.vertices[3 * vi[k] + 1]];
             n[k] = lo.normals[lo.vertices[3 * vi[k] + 2]];
             d[k] = dirs ? dirs->normals[dirs->vertices[3 * vi[k] + 2]] : n[k];
-            tg[k] = has_tan ? lo.tangents[vi[k]] : mk(0, 0, 0);
-            bt[k] = has_tan ? lo.bitangents[vi[k]] : mk(0, 0, 0);
+            const uint32_t ti = hybrid ? lo.vertices[3 * vi[k] + 2] : vi[k];
+            tg[k] = has_tan ? lo.tangents[ti] : mk(0, 0, 0);
+            bt[k] = has_tan ? lo.bitangents[ti] : mk(0, 0, 0);
             s[k] = {(u[k].x - halfpix.x) * scale.x, (u[k].y - halfpix.y) * scale.y};
         }
-        const uint32_t xMin = std::min(round_u32(fminf(s[0].x, fminf(s[1].x, s[2].x))), W);
-        const uint32_t yMin = std::min(round_u32(fminf(s[0].y, fminf(s[1].y, s[2].y))), H);
-        const uint32_t xMax = std::min(round_u32(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x))), W);
-        const uint32_t yMax = std::min(round_u32(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y))), H);
+        uint32_t xMin, yMin, xMax, yMax;
+        if (!hybrid) {
+            xMin = std::min(round_u32(fminf(s[0].x, fminf(s[1].x, s[2].x))), W);
+            yMin = std::min(round_u32(fminf(s[0].y, fminf(s[1].y, s[2].y))), H);
+            xMax = std::min(round_u32(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x))), W);
+            yMax = std::min(round_u32(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y))), H);
+        } else {
+            auto cl = [](float v) { return (uint32_t)(int64_t)fmaxf(std::roundf(v), 0.0f); };
+            xMin = cl(fminf(s[0].x, fminf(s[1].x, s[2].x)));
+            yMin = cl(fminf(s[0].y, fminf(s[1].y, s[2].y)));
+            xMax = cl(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x)));
+            yMax = cl(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y)));
+        }
         for (uint32_t y = yMin; y <= yMax; ++y)
             for (uint32_t x = xMin; x <= xMax; ++x) {
                 V2 q = {(float)x * pixsize.x + halfpix.x, (float)y * pixsize.y + halfpix.y};
@@ -84,9 +109,18 @@ bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, ui
                 if (b.x >= -0.001f && b.x <= 1 && b.y >= -0.001f && b.y <= 1 && b.z >= -0.001f && b.z <= 1) {
                     const size_t i = (size_t)y * W + x;
                     if (i >= npx) continue;  // the reference writes past the end here (y == H)
-                    st3(out.map_pos.data(), i, blend(p[0], p[1], p[2], b));
-                    st3(out.map_dir.data(), i, normalize(blend(d[0], d[1], d[2], b)));
-                    st3(out.map_nrm.data(), i, normalize(blend(n[0], n[1], n[2], b)));
+                    const V3 pp = blend(p[0], p[1], p[2], b), nn = normalize(blend(n[0], n[1], n[2], b));
+                    V3 dd = normalize(blend(d[0], d[1], d[2], b));
+                    if (hybrid) {
+                        const float td = fminf(fminf(edge_distance(p[0], p[1], pp),
+                                                     fminf(edge_distance(p[0], p[2], pp), edge_distance(p[1], p[2], pp))) /
+                                                   edge,
+                                               1.0f);
+                        dd = normalize(dd * (1.0f - td) + nn * td);
+                    }
+                    st3(out.map_pos.data(), i, pp);
+                    st3(out.map_dir.data(), i, dd);
+                    st3(out.map_nrm.data(), i, nn);
                     st3(out.map_tan.data(), i, normalize(blend(tg[0], tg[1], tg[2], b)));
                     st3(out.map_bit.data(), i, normalize(blend(bt[0], bt[1], bt[2], b)));
                 }
@@ -133,19 +167,20 @@ extern "C" {
 
 const char* fornos_last_error(void) { return g_last_error.c_str(); }
 
-fornos_status fornos_texels_create(const fornos_mesh* lo, const fornos_mesh* dirs, uint32_t W,
-                                   uint32_t H, fornos_texels** out)
+fornos_status fornos_texels_create_hybrid(const fornos_mesh* lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                                          float edge, fornos_texels** out)
 {
     if (!lo || !out) return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: null argument");
     if (W == 0 || H == 0 || W > 16384 || H > 16384)
         return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: map size must be in [1, 16384]");
     if (dirs && (dirs->nv() != lo->nv()))
         return set_error(FORNOS_INVALID_ARG, "fornos_texels_create: mapping mesh must be a copy of the low-poly mesh");
+    if (!(edge >= 0.0f)) return set_error(FORNOS_INVALID_ARG, "fornos_texels_create_hybrid: edge must be >= 0");
     *out = nullptr;
     fornos_texels* t = new (std::nothrow) fornos_texels();
     if (!t) return set_error(FORNOS_OOM, "fornos_texels_create: out of host memory");
     try {
-        if (!build_texels(*lo, dirs, W, H, *t)) {
+        if (!build_texels(*lo, dirs, W, H, *t, edge)) {
             delete t;
             return set_error(FORNOS_INVALID_ARG, "Low poly mesh is missing texture coordinates or normals information");
         }
@@ -155,6 +190,12 @@ fornos_status fornos_texels_create(const fornos_mesh* lo, const fornos_mesh* dir
     }
     *out = t;
     return ok();
+}
+
+fornos_status fornos_texels_create(const fornos_mesh* lo, const fornos_mesh* dirs, uint32_t W,
+                                   uint32_t H, fornos_texels** out)
+{
+    return fornos_texels_create_hybrid(lo, dirs, W, H, 0.0f, out);
 }
 
 fornos_status fornos_texels_view_get(const fornos_texels* t, fornos_texel_view* v)

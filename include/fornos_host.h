@@ -56,6 +56,12 @@ fornos_status fornos_mesh_view_get(const fornos_mesh *m, fornos_mesh_view *out);
 fornos_status fornos_mesh_compute_vertex_normals(fornos_mesh *m);
 /* Mesh::computeVertexNormalsAggressive (Src/mesh.cpp:680-721). */
 fornos_status fornos_mesh_compute_vertex_normals_aggressive(fornos_mesh *m);
+/* Mesh::computeFaceNormals (Src/mesh.cpp:627-648). */
+fornos_status fornos_mesh_compute_face_normals(fornos_mesh *m);
+/* Mesh::loadFile (Src/mesh.cpp:612-617): Wavefront .obj (loadWavefrontObj,
+ * :235-456) or .ply (loadPly, :515-604) by extension; a parse error, an
+ * index out of range or another extension is an error. */
+fornos_status fornos_mesh_load(const char *path, fornos_mesh **out);
 /* Mesh::computeTangentSpace (Src/mesh.cpp:724-768). */
 fornos_status fornos_mesh_compute_tangent_space(fornos_mesh *m);
 void fornos_mesh_destroy(fornos_mesh *m);
@@ -68,6 +74,12 @@ void fornos_mesh_destroy(fornos_mesh *m);
 typedef struct fornos_texels fornos_texels;
 fornos_status fornos_texels_create(const fornos_mesh *lo, const fornos_mesh *dirs, uint32_t width,
                                    uint32_t height, fornos_texels **out);
+/* MapUV::fromMeshes_Hybrid (Src/compute.cpp:213-345, 365-370; the Hybrid
+ * mapping method): the direction leans from the smooth direction toward the
+ * surface normal within `edge` (FornosParameters_Shared::mappingEdge,
+ * default 0.05) of a triangle edge.  edge = 0 gives fornos_texels_create. */
+fornos_status fornos_texels_create_hybrid(const fornos_mesh *lo, const fornos_mesh *dirs, uint32_t width,
+                                          uint32_t height, float edge, fornos_texels **out);
 fornos_status fornos_texels_view_get(const fornos_texels *t, fornos_texel_view *out);
 /* The uncompressed MapUV arrays (W*H*3 floats each); any pointer may be NULL. */
 fornos_status fornos_texels_export_mapuv(const fornos_texels *t, float *positions,

@@ -40,9 +40,11 @@ def lib():
         sigs = {
             "orc_vertex_normals": (None, [P, u32, P, u32, P, u32, P, P]),
             "orc_vertex_normals_aggressive": (u32, [P, P, u32, P, u32, P, P]),
+            "orc_face_normals": (None, [P, P, u32, P, u32, P, P]),
             "orc_tangent_space": (None, [P, P, P, P, u32, P, u32, P, P]),
             "orc_mapuv": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, P, P, P, P, P]),
             "orc_compress": (u32, [P, u32, P]),
+            "orc_mapuv_edge": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, f32, P, P, P, P, P]),
             "orc_samples": (None, [u32, u32, P, P]),
             "orc_bvh_build": (u32, [P, u32, P, P, P, u32, u32, u32, P, P, P, P, P]),
             "orc_meshmapping": (None, [P, P, u32, u32, P, P, u32, i32, P, P, i32]),
@@ -80,6 +82,16 @@ def vertex_normals(pos, verts, tris):
     return out, vn
 
 
+def face_normals(pos, verts, tris):
+    """(normals [ntri, 3], per-vertex normal index) of Mesh::computeFaceNormals."""
+    pos, verts, tris = _c(pos, np.float32), _c(verts, np.uint32), _c(tris, np.uint32)
+    nt = len(tris)
+    out = np.empty((nt, 3), np.float32)
+    vn = np.empty(len(verts), np.uint32)
+    lib().orc_face_normals(_p(pos), _p(verts), len(verts), _p(tris), nt, _p(out), _p(vn))
+    return out, vn
+
+
 def vertex_normals_aggressive(pos, verts, tris):
     pos, verts, tris = _c(pos, np.float32), _c(verts, np.uint32), _c(tris, np.uint32)
     out = np.empty((3 * len(tris) + 1, 3), np.float32)
@@ -106,6 +118,21 @@ def mapuv(mesh, width, height, dir_normals=None, dir_vert_nrm=None, tangents=Non
     outs = [np.empty((npx, 3), np.float32) for _ in range(5)]
     ok = lib().orc_mapuv(_p(pos), _p(nrm), _p(uv), _p(verts), _p(tris), len(tris), _p(dn), _p(dv),
                          _p(tg), _p(bt), width, height, *[_p(o) for o in outs])
+    if not ok:
+        return None
+    return dict(zip(["positions", "directions", "normals", "tangents", "bitangents"], outs))
+
+
+def mapuv_edge(mesh, width, height, edge, dir_normals=None, dir_vert_nrm=None, tangents=None, bitangents=None):
+    """MapUV::fromMeshes_Hybrid arrays (W*H, 3) x 5, or None."""
+    pos, nrm, uv = _c(mesh["positions"], np.float32), _c(mesh["normals"], np.float32), _c(mesh["texcoords"], np.float32)
+    verts, tris = _c(mesh["vertices"], np.uint32), _c(mesh["triangles"], np.uint32)
+    dn, dv = _c(dir_normals, np.float32), _c(dir_vert_nrm, np.uint32)
+    tg, bt = _c(tangents, np.float32), _c(bitangents, np.float32)
+    npx = width * height
+    outs = [np.empty((npx, 3), np.float32) for _ in range(5)]
+    ok = lib().orc_mapuv_edge(_p(pos), _p(nrm), _p(uv), _p(verts), _p(tris), len(tris), _p(dn), _p(dv),
+                              _p(tg), _p(bt), width, height, float(edge), *[_p(o) for o in outs])
     if not ok:
         return None
     return dict(zip(["positions", "directions", "normals", "tangents", "bitangents"], outs))

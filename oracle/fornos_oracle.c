@@ -88,6 +88,20 @@ ORC_API void orc_vertex_normals(const float *pos, uint32_t np, const uint32_t *v
     for (uint32_t v = 0; v < nv; ++v) out_vert_nrm[v] = verts[3 * v];
 }
 
+/* Mesh::computeFaceNormals (Src/mesh.cpp:627-648): one normal per triangle,
+ * every corner of the triangle re-pointed at it (last triangle wins). */
+ORC_API void orc_face_normals(const float *pos, const uint32_t *verts, uint32_t nv, const uint32_t *tris,
+                              uint32_t ntri, float *out_nrm, uint32_t *out_vert_nrm)
+{
+    for (uint32_t v = 0; v < nv; ++v) out_vert_nrm[v] = verts[3 * v + 2];
+    for (uint32_t t = 0; t < ntri; ++t) {
+        const uint32_t a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+        v3 p0 = ld3(pos, verts[3 * a]), p1 = ld3(pos, verts[3 * b]), p2 = ld3(pos, verts[3 * c]);
+        st3(out_nrm, t, norm3(cross3(sub3(p1, p0), sub3(p2, p0))));
+        out_vert_nrm[a] = out_vert_nrm[b] = out_vert_nrm[c] = t;
+    }
+}
+
 /* Exact-position key for the std::map<Vector3> of Src/mesh.cpp:682-697
  * (operator< at Src/mesh.cpp:38-41 makes -0 and +0 the same key). */
 typedef struct { float x, y, z; uint32_t index; int used; } agg_slot;
@@ -257,6 +271,84 @@ ORC_API int orc_mapuv(const float *pos, const float *nrm, const float *uv, const
                     st3(o_pos, i, add3(add3(mul3s(p[0], b.x), mul3s(p[1], b.y)), mul3s(p[2], b.z)));
                     st3(o_dir, i, norm3(add3(add3(mul3s(d[0], b.x), mul3s(d[1], b.y)), mul3s(d[2], b.z))));
                     st3(o_nrm, i, norm3(add3(add3(mul3s(n[0], b.x), mul3s(n[1], b.y)), mul3s(n[2], b.z))));
+                    st3(o_tan, i, norm3(add3(add3(mul3s(tt[0], b.x), mul3s(tt[1], b.y)), mul3s(tt[2], b.z))));
+                    st3(o_bit, i, norm3(add3(add3(mul3s(bb[0], b.x), mul3s(bb[1], b.y)), mul3s(bb[2], b.z))));
+                }
+            }
+        }
+    }
+    return 1;
+}
+
+/* Src/compute.cpp:195-210: distance from p to the line through e0, e1, and
+ * the smallest of the three edge-line distances. */
+static float edge_distance(v3 e0, v3 e1, v3 p)
+{
+    v3 v = sub3(p, e0), e = sub3(e1, e0);
+    float t = dot3(v, e) / dot3(e, e);
+    v3 q = add3(e0, mul3s(e, t));
+    v3 w = sub3(q, p);
+    return sqrtf(dot3(w, w));
+}
+
+/* MapUV::fromMeshes_Hybrid (Src/compute.cpp:213-320, 365-370): the raster
+ * of orc_mapuv with the Hybrid differences -- bounds (uint32)max(round(.), 0)
+ * without the upper clamp, tangents indexed by normal index, and the ray
+ * direction normalize(dsmooth * (1 - t) + n * t) with
+ * t = min(triangleDistance / edge, 1). */
+ORC_API int orc_mapuv_edge(const float *pos, const float *nrm, const float *uv, const uint32_t *verts,
+                           const uint32_t *tris, uint32_t ntri, const float *dnrm,
+                           const uint32_t *dvert_nrm, const float *tan, const float *bitan, uint32_t W,
+                           uint32_t H, float edge, float *o_pos, float *o_dir, float *o_nrm, float *o_tan,
+                           float *o_bit)
+{
+    size_t npx = (size_t)W * H;
+    memset(o_pos, 0, sizeof(float) * 3 * npx); memset(o_dir, 0, sizeof(float) * 3 * npx);
+    memset(o_nrm, 0, sizeof(float) * 3 * npx); memset(o_tan, 0, sizeof(float) * 3 * npx);
+    memset(o_bit, 0, sizeof(float) * 3 * npx);
+    v2 scale = {(float)W, (float)H};
+    v2 pixsize = {1.0f / scale.x, 1.0f / scale.y};
+    v2 halfpix = {pixsize.x * 0.5f, pixsize.y * 0.5f};
+    for (uint32_t t = 0; t < ntri; ++t) {
+        uint32_t vi[3] = {tris[3 * t], tris[3 * t + 1], tris[3 * t + 2]};
+        for (int k = 0; k < 3; ++k)
+            if (verts[3 * vi[k] + 1] == UINT32_MAX || verts[3 * vi[k] + 2] == UINT32_MAX) return 0;
+        v3 p[3], n[3], d[3], tt[3], bb[3];
+        v2 u[3];
+        for (int k = 0; k < 3; ++k) {
+            p[k] = ld3(pos, verts[3 * vi[k]]);
+            u[k].x = uv[2 * verts[3 * vi[k] + 1]]; u[k].y = uv[2 * verts[3 * vi[k] + 1] + 1];
+            n[k] = ld3(nrm, verts[3 * vi[k] + 2]);
+            d[k] = dnrm ? ld3(dnrm, dvert_nrm[vi[k]]) : n[k];
+            tt[k] = tan ? ld3(tan, verts[3 * vi[k] + 2]) : mk3(0, 0, 0);
+            bb[k] = tan ? ld3(bitan, verts[3 * vi[k] + 2]) : mk3(0, 0, 0);
+        }
+        v2 s[3];
+        for (int k = 0; k < 3; ++k) {
+            s[k].x = (u[k].x - halfpix.x) * scale.x;
+            s[k].y = (u[k].y - halfpix.y) * scale.y;
+        }
+        uint32_t xMin = (uint32_t)(int64_t)fmaxf(roundf(fminf(s[0].x, fminf(s[1].x, s[2].x))), 0.0f);
+        uint32_t yMin = (uint32_t)(int64_t)fmaxf(roundf(fminf(s[0].y, fminf(s[1].y, s[2].y))), 0.0f);
+        uint32_t xMax = (uint32_t)(int64_t)fmaxf(roundf(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x))), 0.0f);
+        uint32_t yMax = (uint32_t)(int64_t)fmaxf(roundf(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y))), 0.0f);
+        for (uint32_t y = yMin; y <= yMax; ++y) {
+            for (uint32_t x = xMin; x <= xMax; ++x) {
+                v2 q = {(float)x * pixsize.x + halfpix.x, (float)y * pixsize.y + halfpix.y};
+                v3 b = bary2(q, u[0], u[1], u[2]);
+                if (b.x >= -0.001f && b.x <= 1 && b.y >= -0.001f && b.y <= 1 && b.z >= -0.001f &&
+                    b.z <= 1) {
+                    size_t i = (size_t)y * W + x;
+                    if (i >= npx) continue;  /* the reference writes past the end */
+                    v3 pp = add3(add3(mul3s(p[0], b.x), mul3s(p[1], b.y)), mul3s(p[2], b.z));
+                    v3 nn = norm3(add3(add3(mul3s(n[0], b.x), mul3s(n[1], b.y)), mul3s(n[2], b.z)));
+                    float tdist = fminf(fminf(edge_distance(p[0], p[1], pp),
+                                              fminf(edge_distance(p[0], p[2], pp), edge_distance(p[1], p[2], pp))) / edge,
+                                        1.0f);
+                    v3 ds = norm3(add3(add3(mul3s(d[0], b.x), mul3s(d[1], b.y)), mul3s(d[2], b.z)));
+                    st3(o_pos, i, pp);
+                    st3(o_dir, i, norm3(add3(mul3s(ds, 1.0f - tdist), mul3s(nn, tdist))));
+                    st3(o_nrm, i, nn);
                     st3(o_tan, i, norm3(add3(add3(mul3s(tt[0], b.x), mul3s(tt[1], b.y)), mul3s(tt[2], b.z))));
                     st3(o_bit, i, norm3(add3(add3(mul3s(bb[0], b.x), mul3s(bb[1], b.y)), mul3s(bb[2], b.z))));
                 }

@@ -71,6 +71,46 @@ def test_mapuv_and_compress(prep):
     assert eq_nan(m["directions"][idx], ta["directions"])
 
 
+@pytest.mark.parametrize("edge", [0.05, 0.3])
+def test_hybrid_mapuv_matches_oracle(prep, edge):
+    # MapUV::fromMeshes_Hybrid: the Hybrid mapping method
+    lo, d = prep["low"].arrays(), prep["dirs"].arrays()
+    W = prep["texels"].width // 2
+    m = orc.mapuv_edge(lo, W, W, edge, d["normals"], d["vertices"][:, 2], lo.get("tangents"), lo.get("bitangents"))
+    tex = fa.CompressedMapUV(prep["low"], prep["dirs"], W, W, edge=edge)
+    pm = tex.mapuv()
+    for k in m:
+        assert eq_nan(m[k], pm[k]), k
+
+
+def test_hybrid_oracle_limits(prep):
+    # pins of the oracle's blend d = normalize(dsmooth (1 - t) + n t) on a
+    # faceted low-poly mesh (n = face normal, dsmooth = aggressive vertex
+    # normal): a vanishing edge width gives the face normal everywhere
+    # (t = 1), a huge one the Smooth mapping's direction (t ~ 0); positions
+    # and normals do not depend on the width
+    lo, d = dict(prep["low"].arrays()), prep["dirs"].arrays()
+    fn, fv = orc.face_normals(lo["positions"], lo["vertices"], lo["triangles"])
+    lo["normals"] = fn
+    lo["vertices"] = lo["vertices"].copy()
+    lo["vertices"][:, 2] = fv
+    W = 64
+    args = (d["normals"], d["vertices"][:, 2])
+    sharp = orc.mapuv_edge(lo, W, W, 1e-30, *args)
+    soft = orc.mapuv_edge(lo, W, W, 1e30, *args)
+    smooth = orc.mapuv(lo, W, W, *args)
+    hit = np.abs(sharp["normals"]).sum(1) > 0
+    assert hit.sum() > W * W // 4
+    # texels exactly on an edge have distance 0, so t = 0 there at any width
+    near_n = np.abs(sharp["directions"] - sharp["normals"]).max(1) <= 1e-6
+    near_s = np.abs(sharp["directions"] - smooth["directions"]).max(1) <= 1e-6
+    assert np.all(near_n[hit] | near_s[hit]) and near_n[hit].mean() > 0.98
+    assert np.allclose(soft["directions"], smooth["directions"], atol=1e-6)
+    assert np.array_equal(sharp["positions"], soft["positions"]) and np.array_equal(sharp["normals"], soft["normals"])
+    # the two limits differ on this mesh, so neither check above is vacuous
+    assert np.abs(soft["directions"][hit] - sharp["directions"][hit]).max() > 1e-2
+
+
 def test_reference_bvh_matches_oracle(prep):
     hi = prep["high"]
     n = C.c_uint32()

@@ -1,0 +1,164 @@
+"""Mesh files (Mesh::loadFile, Src/mesh.cpp:235-617): the library's OBJ and
+PLY readers against the oracle's restatement (oracle/meshio.py, numbers read
+with the C library's strtod / strtol as the reference reads them), error
+cases, face normals against the C oracle, and a generator mesh round trip."""
+import struct
+
+import numpy as np
+import pytest
+
+import fornos_amd as fa
+import oracle as orc
+from fornos_amd import _abi
+from oracle import meshio as mo
+
+
+def same(mesh, ref):
+    a = mesh.arrays()
+    for k in ("positions", "texcoords", "normals", "vertices", "triangles"):
+        x = np.asarray(a[k]).reshape(ref[k].shape) if a[k].size else np.zeros(ref[k].shape, ref[k].dtype)
+        assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                              ref[k].view(np.uint32) if ref[k].dtype == np.float32 else ref[k]), k
+
+
+OBJ_TEXT = """# a quad, a triangle and a pentagon
+v 0 0 0
+v 1 0 0 1.0
+   v 1 1 0
+v 0 1 0\r
+v 0.5 2e0 -0.25
+v 2 2 2
+vt 0 0
+vt 1 0 0
+vt 1 1
+vt 0 1
+vn 0 0 1
+vp 0.5
+f 1/1/1 2/2/1 3/3/1 4/4/1
+f 4/4/1 3/3/1 5/1/1
+f 1/1/1 2/2/1 3/3/1 5/4/1 6/1/1\r
+l 1 2
+o object_name
+"""
+
+
+def test_obj_matches_oracle(tmp_path):
+    p = tmp_path / "m.obj"
+    p.write_text(OBJ_TEXT)
+    ref = mo.load_obj(str(p))
+    assert len(ref["triangles"]) == 2 + 1 + 3
+    same(fa.Mesh.load(p), ref)
+
+
+@pytest.mark.parametrize("text", ["v 1 2 3\nv 1 2 3\nv 1 2 3\nf 1//1 2//1 3//1\nvn 0 0 1\n",
+                                  "v 1 2 3\nv 4 5 6\nv 7 8 9\nf 1 2 3\nf 3 2 1\n"])
+def test_obj_face_forms(tmp_path, text):
+    p = tmp_path / "m.obj"
+    p.write_text(text)
+    same(fa.Mesh.load(p), mo.load_obj(str(p)))
+
+
+@pytest.mark.parametrize("text", ["v 1 2 3 x\n", "v 1 2\n", "vt 1\n", "v 1 1 1\nv 2 2 2\nv 3 3 3\nvt 0 0\nf 1/1 2 3\n",
+                                  "v 1e999 0 0\n", "v 1 1 1\nf 1 2 3\n", "v 1 1 1\nv 2 2 2\nv 3 3 3\nf 1 2 -1\n",
+                                  "v 1 1 1\nf 1 1 1 # c\n"])
+def test_obj_errors(tmp_path, text):
+    p = tmp_path / "m.obj"
+    p.write_text(text)
+    with pytest.raises(fa.FornosError):
+        fa.Mesh.load(p)
+    # the oracle rejects the same files, except index ranges the reference never checks
+    if "f 1 2 3\n" not in text and "-1" not in text:
+        with pytest.raises(mo.ParseError):
+            mo.load_obj(str(p))
+
+
+def write_ply(path, fmt, pos, faces, st=None, vn=None, fn=None, coord="float", cnt="uchar", idx="int"):
+    props = [("x", coord), ("y", coord), ("z", coord)]
+    if st is not None:
+        props += [("s", "float"), ("t", "float")]
+    if vn is not None:
+        props += [("nx", "float"), ("ny", "float"), ("nz", "float")]
+    head = ["ply", "format %s 1.0" % fmt, "comment test", "element vertex %d" % len(pos)]
+    head += ["property %s %s" % (t, n) for n, t in props]
+    head += ["element face %d" % len(faces), "property list %s %s vertex_indices" % (cnt, idx)]
+    if fn is not None:
+        head += ["property float nx", "property float ny", "property float nz"]
+    head += ["end_header"]
+    rows = []
+    for i, p in enumerate(pos):
+        r = list(p) + (list(st[i]) if st is not None else []) + (list(vn[i]) if vn is not None else [])
+        rows.append(r)
+    with open(path, "wb") as f:
+        f.write(("\n".join(head) + "\n").encode())
+        if fmt == "ascii":
+            for r in rows:
+                f.write((" ".join("%.9g" % v for v in r) + "\n").encode())
+            for i, fc in enumerate(faces):
+                extra = list(fn[i]) if fn is not None else []
+                f.write((" ".join([str(len(fc))] + [str(v) for v in fc] + ["%.9g" % v for v in extra]) + "\n").encode())
+        else:
+            bo = "<" if fmt == "binary_little_endian" else ">"
+            code = {"float": "f", "double": "d", "uchar": "B", "int": "i", "uint": "I", "ushort": "H"}
+            for r in rows:
+                for (n, t), v in zip(props, r):
+                    f.write(struct.pack(bo + code[t], v))
+            for i, fc in enumerate(faces):
+                f.write(struct.pack(bo + code[cnt], len(fc)))
+                f.write(struct.pack(bo + code[idx] * len(fc), *fc))
+                if fn is not None:
+                    f.write(struct.pack(bo + "fff", *fn[i]))
+
+
+@pytest.mark.parametrize("fmt,coord,idx,normals", [("ascii", "float", "int", "vertex"),
+                                                   ("binary_little_endian", "double", "uint", "face"),
+                                                   ("binary_big_endian", "float", "ushort", "none"),
+                                                   ("binary_little_endian", "float", "int", "vertex")])
+def test_ply_matches_oracle(tmp_path, fmt, coord, idx, normals):
+    rng = np.random.default_rng(4)
+    pos = rng.standard_normal((9, 3)).astype(np.float32)
+    faces = [[0, 1, 2], [2, 3, 4, 5], [5, 6, 7], [8, 7, 6, 5, 4]]
+    st = rng.random((9, 2)).astype(np.float32)
+    vn = rng.standard_normal((9, 3)).astype(np.float32) if normals == "vertex" else None
+    fn = rng.standard_normal((4, 3)).astype(np.float32) if normals == "face" else None
+    p = tmp_path / "m.ply"
+    write_ply(p, fmt, pos, faces, st=st, vn=vn, fn=fn, coord=coord, idx=idx)
+    ref = mo.load_ply(str(p))
+    assert len(ref["triangles"]) == 1 + 2 + 1 + 3
+    same(fa.Mesh.load(p), ref)
+
+
+def test_face_normals_match_oracle():
+    m = fa.Mesh.generate(_abi.GEN_ICOSPHERE_WAVES, 3, 0.02, 3, 0)
+    a = m.arrays()
+    m.computeFaceNormals()
+    b = m.arrays()
+    n, vn = orc.face_normals(a["positions"], a["vertices"], a["triangles"])
+    assert np.array_equal(b["normals"].view(np.uint32), n.view(np.uint32))
+    assert np.array_equal(b["vertices"][:, 2], vn)
+
+
+def test_generator_mesh_round_trips_through_obj(tmp_path):
+    m = fa.Mesh.generate(_abi.GEN_TORUS_KNOT, 0, 0.0, 32, 8)
+    a = m.arrays()
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in a["positions"]]
+    lines += ["vt %.9g %.9g" % tuple(t) for t in a["texcoords"]]
+    lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in a["normals"]]
+    V = a["vertices"]
+    for t in a["triangles"]:
+        lines.append("f " + " ".join("%d/%d/%d" % (V[v, 0] + 1, V[v, 1] + 1, V[v, 2] + 1) for v in t))
+    p = tmp_path / "knot.obj"
+    p.write_text("\n".join(lines) + "\n")
+    b = fa.Mesh.load(p).arrays()
+    # per-triangle corners carry the same positions, texcoords and normals
+    def corners(x):
+        v = x["vertices"][x["triangles"].reshape(-1)]
+        return x["positions"][v[:, 0]], x["texcoords"][v[:, 1]], x["normals"][v[:, 2]]
+    for u, w in zip(corners(a), corners(b)):
+        assert np.array_equal(u.view(np.uint32), w.view(np.uint32))
+
+
+def test_unknown_extension_is_an_error(tmp_path):
+    p = tmp_path / "m.stl"
+    p.write_text("solid")
+    with pytest.raises(fa.FornosError):
+        fa.Mesh.load(p)
