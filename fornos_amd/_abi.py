@@ -1,4 +1,4 @@
-"""ctypes binding of libfornos_hip.so (include/fornos_hip.h, include/fornos_host.h).
+"""ctypes binding of libfornos_hip.so (include/fornos_*.h).
 
 The library is built in-tree by fornos_amd.build; loading fails loudly when it
 is missing, so no code path silently runs without the native library.
@@ -53,6 +53,31 @@ class GenParams(C.Structure):
     _fields_ = [("a", C.c_float), ("b", C.c_uint32), ("c", C.c_uint32)]
 
 
+class ParamsShared(C.Structure):
+    """fornos_params_shared (include/fornos_runner.h)."""
+    _fields_ = [("lo_poly_mesh_path", C.c_char_p), ("hi_poly_mesh_path", C.c_char_p),
+                ("lo_poly_mesh_normal", C.c_int), ("hi_poly_mesh_normal", C.c_int),
+                ("bvh_tris_per_node", C.c_int), ("tex_width", C.c_int), ("tex_height", C.c_int),
+                ("tex_dilation", C.c_int), ("ignore_backfaces", C.c_int), ("mapping", C.c_int),
+                ("mapping_edge", C.c_float), ("sample_seed", C.c_uint32)]
+
+
+class ParamsMap(C.Structure):
+    """fornos_params_map (include/fornos_runner.h)."""
+    _fields_ = [("enabled", C.c_int), ("output_path", C.c_char_p), ("sample_count", C.c_int),
+                ("min_distance", C.c_float), ("max_distance", C.c_float), ("tangent_space", C.c_int),
+                ("normalize_output", C.c_int)]
+
+
+class Parameters(C.Structure):
+    """fornos_parameters (include/fornos_runner.h)."""
+    _fields_ = [("shared", ParamsShared), ("height", ParamsMap), ("positions", ParamsMap), ("normals", ParamsMap),
+                ("ao", ParamsMap), ("bent_normals", ParamsMap), ("thickness", ParamsMap)]
+
+
+NORMALS_IMPORT, NORMALS_PER_FACE, NORMALS_PER_VERTEX = range(3)
+MAPPING_SMOOTH, MAPPING_LOWPOLY_NORMALS, MAPPING_HYBRID = range(3)
+
 H = C.c_void_p
 P = C.c_void_p
 u32p = C.POINTER(C.c_uint32)
@@ -89,11 +114,15 @@ _SIGS = [
     ("fornos_meshmap_create", None, [H, C.POINTER(TexelView), C.c_int, C.POINTER(H)]),
     ("fornos_meshmap_create_from_mesh", None, [H, C.POINTER(MeshView), C.POINTER(MeshView), C.c_uint32,
                                                 C.c_uint32, C.c_int, C.POINTER(H)]),
+    ("fornos_meshmap_create_from_mesh_hybrid", None, [H, C.POINTER(MeshView), C.POINTER(MeshView), C.c_uint32,
+                                                       C.c_uint32, C.c_float, C.c_int, C.POINTER(H)]),
     ("fornos_meshmap_read_texels", None, [H, P, P, P, P, P, P]),
     ("fornos_meshmap_run_step", None, [H, C.POINTER(C.c_int)]),
     ("fornos_meshmap_run_all", None, [H, C.POINTER(C.c_float)]),
     ("fornos_meshmap_progress", None, [H, C.POINTER(C.c_float)]),
     ("fornos_meshmap_reset", None, [H]),
+    ("fornos_meshmap_submit", None, [H, C.c_uint32]),
+    ("fornos_meshmap_poll", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     ("fornos_meshmap_read", None, [H, P, P]),
     ("fornos_meshmap_texel_count", None, [H, u32p]),
     ("fornos_meshmap_destroy", "void", [H]),
@@ -102,6 +131,8 @@ _SIGS = [
     ("fornos_solver_run_all", None, [H, C.POINTER(C.c_float)]),
     ("fornos_solver_progress", None, [H, C.POINTER(C.c_float)]),
     ("fornos_solver_reset", None, [H]),
+    ("fornos_solver_submit", None, [H, C.c_uint32]),
+    ("fornos_solver_poll", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     ("fornos_solver_read", None, [H, P, C.c_size_t]),
     ("fornos_solver_read_counts", None, [H, P]),
     ("fornos_solver_last_timing", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_float), u32p]),
@@ -146,6 +177,15 @@ _SIGS = [
     ("fornos_write_tga", None, [C.c_char_p, C.c_uint32, C.c_uint32, P]),
     ("fornos_write_exr", None, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
     ("fornos_float_to_half", "void", [P, P, C.c_size_t]),
+    ("fornos_parameters_defaults", "void", [C.POINTER(Parameters)]),
+    ("fornos_runner_create", None, [C.POINTER(H)]),
+    ("fornos_runner_start", None, [H, C.POINTER(Parameters)]),
+    ("fornos_runner_pending", C.c_int, [H]),
+    ("fornos_runner_run", None, [H]),
+    ("fornos_runner_run_all", None, [H]),
+    ("fornos_runner_current_task", None, [H, C.POINTER(C.c_char_p), C.POINTER(C.c_float)]),
+    ("fornos_runner_exported_range", C.c_int, [H, C.c_int, P]),
+    ("fornos_runner_destroy", "void", [H]),
     ("fornos_dn_conv3x3", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P,
                                  C.c_uint32, C.c_int, P]),
     ("fornos_dn_conv3x3_wgrad", None, [P, C.c_uint32, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,

@@ -227,15 +227,16 @@ class MeshMapping(_Handle):
         check(lib().fornos_meshmap_create(scene.handle, C.byref(tv), int(bool(cullBackfaces)), C.byref(self._h)))
         return self
 
-    def init_from_mesh(self, lo, dirs, width, height, scene, cullBackfaces=False):
-        """MeshMapping::init with the texel set rasterised on the device."""
+    def init_from_mesh(self, lo, dirs, width, height, scene, cullBackfaces=False, edge=0.0):
+        """MeshMapping::init with the texel set rasterised on the device;
+        edge > 0: the Hybrid mapping method (MapUV::fromMeshes_Hybrid)."""
         self.close()
         self.texels, self.scene = None, scene
         lv = lo.mesh_view()
         dv = dirs.mesh_view() if dirs is not None else None
-        check(lib().fornos_meshmap_create_from_mesh(scene.handle, C.byref(lv), C.byref(dv) if dv else None,
-                                                    int(width), int(height), int(bool(cullBackfaces)),
-                                                    C.byref(self._h)))
+        check(lib().fornos_meshmap_create_from_mesh_hybrid(scene.handle, C.byref(lv), C.byref(dv) if dv else None,
+                                                           int(width), int(height), float(edge),
+                                                           int(bool(cullBackfaces)), C.byref(self._h)))
         return self
 
     def read_texels(self):

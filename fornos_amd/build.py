@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 
-HOST_SOURCES = ["mesh.cpp", "texels.cpp", "bvh.cpp", "fornos.cpp", "imageio.cpp", "meshio.cpp"]
+HOST_SOURCES = ["mesh.cpp", "texels.cpp", "bvh.cpp", "fornos.cpp", "imageio.cpp", "meshio.cpp", "runner.cpp"]
 HIP_SOURCES = ["kernels.hip", "denoise.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",

@@ -1,13 +1,21 @@
-"""Seeded bake workloads and the host preparation a bake runs before the GPU.
+"""Bake workloads and the host preparation a bake runs before the GPU.
 
-prepare() restates FornosRunner::start's assembly order (Src/fornos.cpp:53-236)
-for the procedural meshes: low-poly normals as generated ("Import"), the
-tangent frame when a tangent-space map is baked, the Smooth mapping method
-(a copy of the low-poly mesh with aggressive vertex normals gives the ray
-directions), MapUV -> CompressedMapUV, then the hi-poly BVH.  Parameters
-that a workload does not set take the reference defaults (Src/fornos.h:38-123):
-minDistance 0.01, maxDistance 10, bvhTrisPerNode 8, ignoreBackfaces true.
+prepare() restates FornosRunner::start's assembly order (Src/fornos.cpp:53-236):
+the low-poly mesh (a seeded generator or an .obj / .ply file) and its normal
+import mode, the hi-poly mesh (or the low-poly one) and its mode, the
+tangent frame when a tangent-space map is baked, the mapping mesh of the
+method (Smooth and Hybrid: a copy of the low-poly mesh with aggressive
+vertex normals unless the low-poly normals are computed per vertex;
+LowPolyNormals: none), MapUV -> CompressedMapUV, then the hi-poly BVH.
+
+Workload keys beyond the meshes, the map size and the maps, with the
+reference defaults (Src/fornos.h:38-123): "lowNormals" / "highNormals"
+("import", "face", "vertex"; import), "mapping" ("smooth", "lowpoly",
+"hybrid"; smooth), "mappingEdge" (0.05), "width" / "height" (else "size"),
+"bvhTrisPerNode" (8), "cull" (ignoreBackfaces, true).
 """
+import os
+
 from . import _abi
 from .api import CompressedMapUV, Mesh, Scene
 
@@ -62,8 +70,29 @@ def scaled(name, size=None, high_quads=None):
 
 
 def make_mesh(spec):
+    """A generator tuple (generator, seed, a, b, c) or a mesh file path."""
+    if isinstance(spec, (str, os.PathLike)):
+        return Mesh.load(spec)
     gen, seed, a, b, c = spec
     return Mesh.generate(gen, seed, a, b, c)
+
+
+NORMAL_MODES = ("import", "face", "vertex")
+MAPPINGS = ("smooth", "lowpoly", "hybrid")
+
+
+def apply_normals(mesh, mode):
+    """NormalImport (Src/fornos.cpp:59-64)."""
+    if mode not in NORMAL_MODES:
+        raise ValueError("normal import mode must be one of %s, not %r" % (NORMAL_MODES, mode))
+    if mode == "face":
+        mesh.computeFaceNormals()
+    elif mode == "vertex":
+        mesh.computeVertexNormals()
+
+
+def tex_size(cfg):
+    return int(cfg.get("width", cfg["size"])), int(cfg.get("height", cfg["size"]))
 
 
 def needs_tangent_space(cfg):
@@ -76,14 +105,35 @@ def prepare(cfg, with_scene=True):
     """Meshes, texel set and (optionally) the device scene of a workload."""
     if isinstance(cfg, str):
         cfg = CONFIGS[cfg]
+    lo_mode = cfg.get("lowNormals", "import")
     lo = make_mesh(cfg["low"])
-    hi = make_mesh(cfg["high"])
+    apply_normals(lo, lo_mode)
+    if cfg.get("high") is None:
+        hi = lo
+    else:
+        hi = make_mesh(cfg["high"])
+        apply_normals(hi, cfg.get("highNormals", "import"))
     if needs_tangent_space(cfg):
         lo.computeTangentSpace()
-    dirs = lo.copy()
-    dirs.computeVertexNormalsAggressive()
-    texels = CompressedMapUV(lo, dirs, cfg["size"], cfg["size"])
-    out = {"cfg": cfg, "low": lo, "dirs": dirs, "high": hi, "texels": texels}
+    mapping = cfg.get("mapping", "smooth")
+    if mapping not in MAPPINGS:
+        raise ValueError("mapping must be one of %s, not %r" % (MAPPINGS, mapping))
+    edge = 0.0
+    if mapping == "hybrid":
+        edge = float(cfg.get("mappingEdge", 0.05))
+        if not edge > 0:
+            raise ValueError("the Hybrid mapping needs mappingEdge > 0")
+    if mapping == "lowpoly":
+        dirs = None
+    elif lo_mode == "vertex":
+        dirs = lo
+    else:
+        dirs = lo.copy()
+        dirs.computeVertexNormalsAggressive()
+    W, H = tex_size(cfg)
+    texels = CompressedMapUV(lo, dirs, W, H, edge=edge)
+    out = {"cfg": cfg, "low": lo, "dirs": dirs, "high": hi, "texels": texels, "width": W, "height": H,
+           "edge": edge}
     if with_scene:
-        out["scene"] = Scene(hi)
+        out["scene"] = Scene(hi, bvh_tris_per_node=int(cfg.get("bvhTrisPerNode", 8)))
     return out

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "host.h"
 
@@ -81,6 +82,46 @@ struct Timer {
 };
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Completion marks of work submitted without waiting: an event recorded
+// after each part, with the amount of work finished once it has completed.
+// poll() queries the events in order and never blocks.
+struct Marks {
+    std::vector<hipEvent_t> ev;
+    std::vector<uint64_t> work;  // work finished when ev[i] has completed
+    size_t reached = 0;          // leading events known to have completed
+    ~Marks() { clear(); }
+    void clear()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        work.clear();
+        reached = 0;
+    }
+    bool empty() const { return ev.empty(); }
+    bool complete() const { return reached == ev.size(); }
+    fornos_status add(hipStream_t st, uint64_t w)
+    {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev.push_back(e);
+        work.push_back(w);
+        HIP_TRY(hipEventRecord(e, st));
+        return FORNOS_OK;
+    }
+    // Work finished so far (0 before the first mark).
+    fornos_status poll(uint64_t* finished)
+    {
+        while (reached < ev.size()) {
+            const hipError_t r = hipEventQuery(ev[reached]);
+            if (r == hipErrorNotReady) break;
+            HIP_TRY(r);
+            ++reached;
+        }
+        *finished = reached ? work[reached - 1] : 0;
+        return FORNOS_OK;
+    }
+};
 
 }  // namespace
 }  // namespace fornos

@@ -45,8 +45,9 @@ void tangent_space(fornos_mesh& m);              // Src/mesh.cpp:724-768
 void face_normals(fornos_mesh& m);               // Src/mesh.cpp:627-648 (meshio.cpp)
 
 // texels.cpp
-bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
-                  fornos_texels& out, float edge = 0.0f);
+// nullptr on success, else the reason
+const char* build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                         fornos_texels& out, float edge = 0.0f);
 
 // Reference-layout BVH, flattened (bvh.cpp).
 struct FlatBVH {

@@ -61,6 +61,8 @@ struct fornos_meshmap {
     uint32_t work_count = 0;  // n rounded up to 64, Src/meshmapping.cpp:165
     uint32_t offset = 0;      // texels mapped so far
     int cull = 1;
+    Marks marks;              // parts of the last fornos_meshmap_submit
+    uint32_t mark_from = 0;   // texel its first part started at
     float4 *pix = nullptr, *pixt = nullptr, *coords = nullptr;
     uint32_t *tidx = nullptr, *indices = nullptr;
     ~fornos_meshmap()
@@ -88,6 +90,8 @@ struct fornos_solver {
     // per-launch events of the last timed run: (before k_rays, after k_rays, after k_reduce)
     std::vector<hipEvent_t> events;
     bool timed = false;
+    Marks marks;              // parts of the last fornos_solver_submit
+    uint32_t mark_from = 0;   // texel its first part started at
     float rays_ms = 0, reduce_ms = 0;
     uint32_t launches = 0;
     ~fornos_solver()
@@ -414,12 +418,21 @@ fornos_status fornos_meshmap_create_from_mesh(fornos_scene* sc, const fornos_mes
                                               const fornos_mesh_view* dirs, uint32_t W, uint32_t H, int cull,
                                               fornos_meshmap** out)
 {
+    return fornos_meshmap_create_from_mesh_hybrid(sc, lo, dirs, W, H, 0.0f, cull, out);
+}
+
+fornos_status fornos_meshmap_create_from_mesh_hybrid(fornos_scene* sc, const fornos_mesh_view* lo,
+                                                     const fornos_mesh_view* dirs, uint32_t W, uint32_t H,
+                                                     float edge, int cull, fornos_meshmap** out)
+{
     if (!sc || !lo || !out) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: null argument");
     *out = nullptr;
     if (W == 0 || H == 0 || W > 16384 || H > 16384)
         return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: map size must be in [1, 16384]");
     if (dirs && dirs->n_vertices != lo->n_vertices)
         return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh: mapping mesh must be a copy of the low-poly mesh");
+    if (!(edge >= 0.0f)) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create_from_mesh_hybrid: edge must be >= 0");
+    const bool hybrid = edge > 0.0f;
     // corner records of rasterTriangle (Src/compute.cpp:85-126)
     const uint32_t nt = lo->n_triangles;
     std::vector<float> tris((size_t)kTriFloats * nt, 0.0f);
@@ -444,9 +457,20 @@ fornos_status fornos_meshmap_create_from_mesh(fornos_scene* sc, const fornos_mes
             }
             std::memcpy(c + 8, d, 12);
             if (lo->tangents && lo->bitangents) {
-                std::memcpy(c + 11, &lo->tangents[3 * (size_t)v], 12);
-                std::memcpy(c + 14, &lo->bitangents[3 * (size_t)v], 12);
+                // the Hybrid raster reads the tangent frame by normal index (Src/compute.cpp:264-269)
+                const uint32_t tv = hybrid ? ni : v;
+                if (tv >= lo->n_vertices)
+                    return set_error(FORNOS_INVALID_ARG, "Hybrid mapping: a normal index is out of the tangent frame's range");
+                std::memcpy(c + 11, &lo->tangents[3 * (size_t)tv], 12);
+                std::memcpy(c + 14, &lo->bitangents[3 * (size_t)tv], 12);
             }
+        }
+        if (hybrid) {
+            V2 u[3];
+            for (int k = 0; k < 3; ++k) u[k] = {tris[(size_t)t * kTriFloats + k * kCornerFloats + 3],
+                                                tris[(size_t)t * kTriFloats + k * kCornerFloats + 4]};
+            if (!texel_box(u, W, H, true).ok)
+                return set_error(FORNOS_INVALID_ARG, "texture coordinates too far outside [0, 1] for the Hybrid mapping");
         }
     }
     std::unique_ptr<fornos_meshmap> mm(new fornos_meshmap());
@@ -459,7 +483,8 @@ fornos_status fornos_meshmap_create_from_mesh(fornos_scene* sc, const fornos_mes
     struct Tmp {
         float* tris = nullptr;
         float* map = nullptr;
-        uint32_t *owner = nullptr, *flags = nullptr, *slot = nullptr;
+        unsigned long long* owner = nullptr;
+        uint32_t *flags = nullptr, *slot = nullptr;
         void* scan = nullptr;
         ~Tmp() { dev_free(tris); dev_free(map); dev_free(owner); dev_free(flags); dev_free(slot); dev_free(scan); }
     } tmp;
@@ -470,9 +495,9 @@ fornos_status fornos_meshmap_create_from_mesh(fornos_scene* sc, const fornos_mes
     if (fornos_status s = dev_alloc(&tmp.slot, npx)) return s;
     HIP_TRY(hipMemcpyAsync(tmp.tris, tris.data(), sizeof(float) * tris.size(), hipMemcpyHostToDevice, st));
     const uint32_t pblocks = div_up(npx, kBlock);
-    hipLaunchKernelGGL(k_fill, dim3(pblocks), dim3(kBlock), 0, st, tmp.owner, (size_t)npx, 0u);
+    hipLaunchKernelGGL(k_fill, dim3(div_up(2 * npx, kBlock)), dim3(kBlock), 0, st, (uint32_t*)tmp.owner, (size_t)(2 * npx), 0u);
     hipLaunchKernelGGL(k_fill, dim3(div_up(15 * npx, kBlock)), dim3(kBlock), 0, st, (uint32_t*)tmp.map, (size_t)(15 * npx), 0u);
-    RasterArgs R{tmp.tris, nt, W, H, tmp.owner, tmp.map};
+    RasterArgs R{tmp.tris, nt, W, H, edge, tmp.owner, tmp.map};
     if (nt) hipLaunchKernelGGL(k_raster_bid, dim3(nt), dim3(kBlock), 0, st, R);
     hipLaunchKernelGGL(k_raster_write, dim3(pblocks), dim3(kBlock), 0, st, R);
     hipLaunchKernelGGL(k_texel_flags, dim3(pblocks), dim3(kBlock), 0, st, tmp.map + 3 * npx, npx, tmp.flags);
@@ -573,6 +598,7 @@ fornos_status fornos_meshmap_reset(fornos_meshmap* mm)
 {
     if (!mm) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_reset: null handle");
     mm->offset = 0;
+    mm->marks.clear();
     return FORNOS_OK;
 }
 
@@ -581,6 +607,50 @@ fornos_status fornos_meshmap_progress(const fornos_meshmap* mm, float* out)
     if (!mm || !out) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_progress: null argument");
     *out = mm->work_count ? (float)mm->offset / (float)mm->work_count : 1.0f;
     return FORNOS_OK;
+}
+
+// Submission without waiting (the runner's pacing): the remaining texels in
+// `parts` launches, an event after each; poll() reports finished texels.
+static fornos_status poll_marks(Marks& marks, hipStream_t st, uint32_t from, uint32_t n, bool submitted,
+                                float* progress, int* done)
+{
+    if (marks.empty()) {
+        const hipError_t r = hipStreamQuery(st);
+        if (r != hipErrorNotReady) HIP_TRY(r);
+        const bool idle = r == hipSuccess;
+        if (progress) *progress = submitted ? 1.0f : 0.0f;
+        if (done) *done = submitted && idle;
+        return FORNOS_OK;
+    }
+    uint64_t fin = 0;
+    if (fornos_status e = marks.poll(&fin)) return e;
+    fin = std::max<uint64_t>(fin, from);
+    if (progress) *progress = marks.complete() ? 1.0f : (n ? (float)fin / (float)n : 1.0f);
+    if (done) *done = marks.complete();
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_submit(fornos_meshmap* mm, uint32_t parts)
+{
+    if (!mm) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_submit: null handle");
+    if (parts == 0 || parts > 4096) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_submit: parts must be in [1, 4096]");
+    mm->marks.clear();
+    const uint32_t t0 = std::min(mm->offset, mm->n);
+    mm->mark_from = t0;
+    const uint32_t per = std::max<uint32_t>(1, div_up(mm->n - t0, parts));
+    for (uint32_t a = t0; a < mm->n; a += per) {
+        const uint32_t b = std::min(mm->n, a + per);
+        if (fornos_status st = meshmap_launch(mm, a, b)) return st;
+        if (fornos_status st = mm->marks.add(mm->sc->stream, b)) return st;
+    }
+    mm->offset = mm->work_count;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_poll(fornos_meshmap* mm, float* progress, int* done)
+{
+    if (!mm) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_poll: null handle");
+    return poll_marks(mm->marks, mm->sc->stream, mm->mark_from, mm->n, mm->offset >= mm->work_count, progress, done);
 }
 
 fornos_status fornos_meshmap_read(const fornos_meshmap* mm, float* coords4, uint32_t* tidx)
@@ -747,10 +817,37 @@ fornos_status fornos_solver_progress(const fornos_solver* s, float* out)
     return FORNOS_OK;
 }
 
+fornos_status fornos_solver_submit(fornos_solver* s, uint32_t parts)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_submit: null handle");
+    if (parts == 0 || parts > 4096) return set_error(FORNOS_INVALID_ARG, "fornos_solver_submit: parts must be in [1, 4096]");
+    s->marks.clear();
+    const uint32_t n = s->mm->n;
+    const uint32_t S = s->hemi() ? s->p.sample_count : 1;
+    const uint32_t t0 = (uint32_t)std::min<uint64_t>(s->offset / S, n);
+    s->mark_from = t0;
+    const uint32_t per = std::max<uint32_t>(1, div_up(n - t0, parts));
+    for (uint32_t a = t0; a < n; a += per) {
+        const uint32_t b = std::min(n, a + per);
+        if (fornos_status st = solver_launch(s, a, b, false)) return st;
+        if (fornos_status st = s->marks.add(s->mm->sc->stream, b)) return st;
+    }
+    s->offset = s->work_total;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_poll(fornos_solver* s, float* progress, int* done)
+{
+    if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_poll: null handle");
+    return poll_marks(s->marks, s->mm->sc->stream, s->mark_from, s->mm->n, s->offset >= s->work_total, progress,
+                      done);
+}
+
 fornos_status fornos_solver_reset(fornos_solver* s)
 {
     if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_reset: null handle");
     s->offset = 0;
+    s->marks.clear();
     return FORNOS_OK;
 }
 

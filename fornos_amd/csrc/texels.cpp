@@ -7,6 +7,7 @@
 #include <random>
 
 #include "host.h"
+#include "mapuv.h"
 
 using namespace fornos;
 
@@ -23,40 +24,21 @@ fornos_status ok() { return FORNOS_OK; }
 
 namespace {
 
-// Src/math.h:250-258
-V3 bary2(V2 p, V2 p0, V2 p1, V2 p2)
-{
-    V2 v0 = {p1.x - p0.x, p1.y - p0.y}, v1 = {p2.x - p0.x, p2.y - p0.y}, v2 = {p.x - p0.x, p.y - p0.y};
-    float s = 1.0f / (v0.x * v1.y - v1.x * v0.y);
-    float i = (v2.x * v1.y - v1.x * v2.y) * s;
-    float j = (v0.x * v2.y - v2.x * v0.y) * s;
-    return mk(1.0f - i - j, i, j);
-}
-
-// uint32_t(std::roundf(x)) as the reference's x86-64 build evaluates it: a
-// 64-bit truncating conversion of which the low 32 bits are kept.
-uint32_t round_u32(float x) { return (uint32_t)(int64_t)std::roundf(x); }
-
 V3 blend(V3 a, V3 b, V3 c, V3 w) { return a * w.x + b * w.y + c * w.z; }
 
 }  // namespace
 
-// Src/compute.cpp:195-210
-float edge_distance(V3 e0, V3 e1, V3 p)
-{
-    const V3 v = p - e0, e = e1 - e0;
-    const float t = dot(v, e) / dot(e, e);
-    const V3 w = (e0 + e * t) - p;
-    return std::sqrt(dot(w, w));
-}
+const char* const kMissingAttributes = "Low poly mesh is missing texture coordinates or normals information";
+const char* const kTangentIndex = "Hybrid mapping: a normal index is out of the tangent frame's range";
+const char* const kUvBoxLimitMsg = "texture coordinates too far outside [0, 1] for the Hybrid mapping";
 
 // Src/compute.cpp:75-193 (edge <= 0: createMapUV, the Smooth and
 // LowPolyNormals methods) or Src/compute.cpp:213-345 (edge > 0:
 // createMapUVEdge, the Hybrid method: bounds clamped at 0 only, tangents
 // indexed by normal index, direction blended toward the surface normal by
 // min(triangleDistance / edge, 1)).
-bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
-                  fornos_texels& out, float edge)
+const char* build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, uint32_t H,
+                         fornos_texels& out, float edge)
 {
     const bool hybrid = edge > 0.0f;
     const size_t npx = (size_t)W * H;
@@ -67,56 +49,37 @@ bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, ui
     out.map_nrm.assign(3 * npx, 0.0f);
     out.map_tan.assign(3 * npx, 0.0f);
     out.map_bit.assign(3 * npx, 0.0f);
-    const V2 scale = {(float)W, (float)H};
-    const V2 pixsize = {1.0f / scale.x, 1.0f / scale.y};
-    const V2 halfpix = {pixsize.x * 0.5f, pixsize.y * 0.5f};
     const bool has_tan = !lo.tangents.empty();
     const uint32_t nt = lo.ntri();
     for (uint32_t t = 0; t < nt; ++t) {
         uint32_t vi[3] = {lo.triangles[3 * t], lo.triangles[3 * t + 1], lo.triangles[3 * t + 2]};
         for (int k = 0; k < 3; ++k)
             if (lo.vertices[3 * vi[k] + 1] == UINT32_MAX || lo.vertices[3 * vi[k] + 2] == UINT32_MAX)
-                return false;
+                return kMissingAttributes;
         V3 p[3], n[3], d[3], tg[3], bt[3];
-        V2 u[3], s[3];
+        V2 u[3];
         for (int k = 0; k < 3; ++k) {
             p[k] = lo.positions[lo.vertices[3 * vi[k]]];
             u[k] = lo.texcoords[lo.vertices[3 * vi[k] + 1]];
             n[k] = lo.normals[lo.vertices[3 * vi[k] + 2]];
             d[k] = dirs ? dirs->normals[dirs->vertices[3 * vi[k] + 2]] : n[k];
             const uint32_t ti = hybrid ? lo.vertices[3 * vi[k] + 2] : vi[k];
+            if (has_tan && ti >= lo.tangents.size()) return kTangentIndex;
             tg[k] = has_tan ? lo.tangents[ti] : mk(0, 0, 0);
             bt[k] = has_tan ? lo.bitangents[ti] : mk(0, 0, 0);
-            s[k] = {(u[k].x - halfpix.x) * scale.x, (u[k].y - halfpix.y) * scale.y};
         }
-        uint32_t xMin, yMin, xMax, yMax;
-        if (!hybrid) {
-            xMin = std::min(round_u32(fminf(s[0].x, fminf(s[1].x, s[2].x))), W);
-            yMin = std::min(round_u32(fminf(s[0].y, fminf(s[1].y, s[2].y))), H);
-            xMax = std::min(round_u32(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x))), W);
-            yMax = std::min(round_u32(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y))), H);
-        } else {
-            auto cl = [](float v) { return (uint32_t)(int64_t)fmaxf(std::roundf(v), 0.0f); };
-            xMin = cl(fminf(s[0].x, fminf(s[1].x, s[2].x)));
-            yMin = cl(fminf(s[0].y, fminf(s[1].y, s[2].y)));
-            xMax = cl(fmaxf(s[0].x, fmaxf(s[1].x, s[2].x)));
-            yMax = cl(fmaxf(s[0].y, fmaxf(s[1].y, s[2].y)));
-        }
-        for (uint32_t y = yMin; y <= yMax; ++y)
-            for (uint32_t x = xMin; x <= xMax; ++x) {
-                V2 q = {(float)x * pixsize.x + halfpix.x, (float)y * pixsize.y + halfpix.y};
-                V3 b = bary2(q, u[0], u[1], u[2]);
-                if (b.x >= -0.001f && b.x <= 1 && b.y >= -0.001f && b.y <= 1 && b.z >= -0.001f && b.z <= 1) {
+        const TexelBox bx = texel_box(u, W, H, hybrid);
+        if (!bx.ok) return kUvBoxLimitMsg;
+        for (uint32_t y = bx.yMin; y <= bx.yMax; ++y)
+            for (uint32_t x = bx.xMin; x <= bx.xMax; ++x) {
+                V3 b;
+                if (texel_inside(u, x, y, W, H, b)) {
                     const size_t i = (size_t)y * W + x;
                     if (i >= npx) continue;  // the reference writes past the end here (y == H)
                     const V3 pp = blend(p[0], p[1], p[2], b), nn = normalize(blend(n[0], n[1], n[2], b));
                     V3 dd = normalize(blend(d[0], d[1], d[2], b));
                     if (hybrid) {
-                        const float td = fminf(fminf(edge_distance(p[0], p[1], pp),
-                                                     fminf(edge_distance(p[0], p[2], pp), edge_distance(p[1], p[2], pp))) /
-                                                   edge,
-                                               1.0f);
-                        dd = normalize(dd * (1.0f - td) + nn * td);
+                        dd = hybrid_direction(p[0], p[1], p[2], pp, nn, dd, edge);
                     }
                     st3(out.map_pos.data(), i, pp);
                     st3(out.map_dir.data(), i, dd);
@@ -143,7 +106,7 @@ bool build_texels(const fornos_mesh& lo, const fornos_mesh* dirs, uint32_t W, ui
         std::memcpy(&out.tan[3 * k], &out.map_tan[3 * i], 12);
         std::memcpy(&out.bit[3 * k], &out.map_bit[3 * i], 12);
     }
-    return true;
+    return nullptr;
 }
 
 }  // namespace fornos
@@ -180,9 +143,9 @@ fornos_status fornos_texels_create_hybrid(const fornos_mesh* lo, const fornos_me
     fornos_texels* t = new (std::nothrow) fornos_texels();
     if (!t) return set_error(FORNOS_OOM, "fornos_texels_create: out of host memory");
     try {
-        if (!build_texels(*lo, dirs, W, H, *t, edge)) {
+        if (const char* err = build_texels(*lo, dirs, W, H, *t, edge)) {
             delete t;
-            return set_error(FORNOS_INVALID_ARG, "Low poly mesh is missing texture coordinates or normals information");
+            return set_error(FORNOS_INVALID_ARG, err);
         }
     } catch (const std::bad_alloc&) {
         delete t;

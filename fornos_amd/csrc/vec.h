@@ -49,6 +49,20 @@ FN V3 hmax(V3 a, V3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, 
 FN float gmin(float x, float y) { return (y < x) ? y : x; }
 FN float gmax(float x, float y) { return (x < y) ? y : x; }
 
+// edgeDistance / triangleDistance (Src/compute.cpp:195-210): distance of p
+// to the line through e0, e1; the smallest over the edges 01, 02, 12.
+FN float edge_distance(V3 e0, V3 e1, V3 p)
+{
+    const V3 v = p - e0, e = e1 - e0;
+    const float t = dot(v, e) / dot(e, e);
+    const V3 w = (e0 + e * t) - p;
+    return sqrtf(dot(w, w));
+}
+FN float triangle_distance(V3 p0, V3 p1, V3 p2, V3 p)
+{
+    return fminf(edge_distance(p0, p1, p), fminf(edge_distance(p0, p2, p), edge_distance(p1, p2, p)));
+}
+
 FN V3 ld3(const float* p, size_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 FN void st3(float* p, size_t i, V3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
 

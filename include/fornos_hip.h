@@ -131,6 +131,16 @@ fornos_status fornos_meshmap_create_from_mesh(fornos_scene *scene, const fornos_
                                               const fornos_mesh_view *dirs, uint32_t width,
                                               uint32_t height, int cull_backfaces,
                                               fornos_meshmap **out);
+/* The same with the Hybrid mapping method, MapUV::fromMeshes_Hybrid
+ * (Src/compute.h:240, Src/compute.cpp:213-349) when edge > 0: ray
+ * directions blended from dirs' normals toward lo's interpolated normal by
+ * t = min(triangleDistance / edge, 1), tangent frame read by normal index,
+ * bounding box clamped at 0 only.  edge == 0 is the call above.  Equals
+ * fornos_texels_create_hybrid bit for bit. */
+fornos_status fornos_meshmap_create_from_mesh_hybrid(fornos_scene *scene, const fornos_mesh_view *lo,
+                                                     const fornos_mesh_view *dirs, uint32_t width,
+                                                     uint32_t height, float edge, int cull_backfaces,
+                                                     fornos_meshmap **out);
 /* Readback of the texel set held by a mesh mapping (3 floats per texel each,
  * plus the MapUV index of each texel); any pointer may be NULL. */
 fornos_status fornos_meshmap_read_texels(const fornos_meshmap *mm, float *positions,
@@ -145,6 +155,17 @@ fornos_status fornos_meshmap_run_all(fornos_meshmap *mm, float *ms);
 /* Rewinds progress to 0 so the next run_step/run_all maps every texel again
  * (the reference re-runs a pass only through a new init). */
 fornos_status fornos_meshmap_reset(fornos_meshmap *mm);
+/* Asynchronous submission of the remaining mapping work as `parts` launches
+ * (1..4096) with a completion mark after each; returns without waiting.
+ * Replaces the reference's one-runStep-per-frame pacing
+ * (Src/fornos.cpp:239-251, MeshMapping::runStep at Src/meshmapping.cpp:186-227):
+ * same results, fewer launches. */
+fornos_status fornos_meshmap_submit(fornos_meshmap *mm, uint32_t parts);
+/* Non-blocking completion query: *progress = finished fraction of the
+ * texels (by completed marks), *done = 1 once all submitted work has
+ * finished on the device.  Without a submission: done when run_step/run_all
+ * has covered all work and the stream is idle. */
+fornos_status fornos_meshmap_poll(fornos_meshmap *mm, float *progress, int *done);
 /* MeshMapping::progress (Src/meshmapping.h:48). */
 fornos_status fornos_meshmap_progress(const fornos_meshmap *mm, float *out);
 /* Readback of r_coords (t, b0, b1, b2 per texel) and r_tidx (soup vertex
@@ -193,6 +214,9 @@ fornos_status fornos_solver_progress(const fornos_solver *s, float *out);
 /* Rewinds progress to 0 so the next run_step/run_all repeats the whole pass
  * (same inputs, same results). */
 fornos_status fornos_solver_reset(fornos_solver *s);
+/* The same pair for a solver (runStep pacing of Src/solver_*.cpp). */
+fornos_status fornos_solver_submit(fornos_solver *s, uint32_t parts);
+fornos_status fornos_solver_poll(fornos_solver *s, float *progress, int *done);
 /* XSolver::getResults (Src/solver_ao.cpp:133-138): copies count values into
  * caller memory: 1 float per texel (height, AO, thickness) or 3 (position,
  * normals, bent normals).  bytes must be at least that size. */

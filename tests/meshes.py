@@ -133,3 +133,20 @@ def merge(*meshes):
         out["triangles"].append(m["triangles"] + vo)
         po += len(m["positions"]); no += len(m["normals"]); to += len(m["texcoords"]); vo += len(m["vertices"])
     return {k: np.concatenate(v).astype(v[0].dtype) for k, v in out.items()}
+
+
+def write_obj(path, a):
+    """Writes mesh arrays (Mesh.arrays() layout) as an OBJ file whose floats
+    read back bit for bit (9 significant digits); faces v/vt/vn, 1-based."""
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in a["positions"]]
+    lines += ["vt %.9g %.9g" % tuple(t) for t in a["texcoords"]]
+    lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in a["normals"]]
+    V = a["vertices"]
+    has_uv = len(a["texcoords"]) > 0
+    for t in a["triangles"]:
+        if has_uv:
+            lines.append("f " + " ".join("%d/%d/%d" % (V[v, 0] + 1, V[v, 1] + 1, V[v, 2] + 1) for v in t))
+        else:
+            lines.append("f " + " ".join("%d//%d" % (V[v, 0] + 1, V[v, 2] + 1) for v in t))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -150,3 +150,41 @@ def test_generators_deterministic():
     c = fa.Mesh.generate(2, 4, 0.05, 48, 20).arrays()
     assert np.array_equal(a["positions"], b["positions"])
     assert not np.array_equal(a["positions"], c["positions"])
+
+
+def _wrap_mesh():
+    # texture coordinates past u = 1: the Hybrid raster does not clamp its
+    # box to the map, so columns x >= W land on following rows; a second,
+    # overlapping triangle overwrites part of the first; tilted normals make
+    # the blend visible
+    from meshes import soup_mesh
+    e = 1.0 / 128
+    tris = [[(0, 0, 0), (1, 0, 0.2), (1, 1, 0)], [(0, 0, 1), (1, 0, 1), (0, 1, 1.3)]]
+    nrm = [[(0, 0, 1), (0.3, 0, 0.95), (0, 0.3, 0.95)], [(0.1, 0, 1), (0, 0, 1), (0, 0.2, 1)]]
+    uvs = [[(e, e), (1.5, e), (1.5, 1 - e)], [(e, e), (1.25, e), (e, 1.0)]]
+    m = soup_mesh(tris, normals=nrm, uvs=uvs)
+    return m, fa.Mesh.from_arrays(m["positions"], m["vertices"], m["triangles"], m["normals"], m["texcoords"])
+
+
+@pytest.mark.parametrize("edge", [0.02, 0.25])
+def test_hybrid_uv_past_the_edge_matches_oracle(edge):
+    m, lo = _wrap_mesh()
+    W, H = 96, 64
+    ref = orc.mapuv_edge(m, W, H, edge)
+    pm = fa.CompressedMapUV(lo, None, W, H, edge=edge).mapuv()
+    for k in ref:
+        assert eq_nan(ref[k], pm[k]), k
+    # columns past the edge did wrap onto the next rows: u in [1, 1.5] is
+    # x in [0.66, 1] on the first triangle, found at the start of rows
+    pos = ref["positions"].reshape(H, W, 3)
+    assert np.any(pos[1:, 0, 0] > 0.6)
+
+
+def test_hybrid_rejects_runaway_uv_boxes():
+    from meshes import soup_mesh
+    m = soup_mesh([[(0, 0, 0), (1, 0, 0), (1, 1, 0)]], uvs=[[(0, 0), (1e6, 0), (1e6, 1)]])
+    lo = fa.Mesh.from_arrays(m["positions"], m["vertices"], m["triangles"], m["normals"], m["texcoords"])
+    with pytest.raises(Exception, match="too far outside"):
+        fa.CompressedMapUV(lo, None, 2048, 2048, edge=0.05)
+    # the Smooth raster clamps its box to the map and takes the same mesh
+    assert fa.CompressedMapUV(lo, None, 64, 64).count >= 0

@@ -139,15 +139,10 @@ def test_face_normals_match_oracle():
 
 def test_generator_mesh_round_trips_through_obj(tmp_path):
     m = fa.Mesh.generate(_abi.GEN_TORUS_KNOT, 0, 0.0, 32, 8)
+    from meshes import write_obj
     a = m.arrays()
-    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in a["positions"]]
-    lines += ["vt %.9g %.9g" % tuple(t) for t in a["texcoords"]]
-    lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in a["normals"]]
-    V = a["vertices"]
-    for t in a["triangles"]:
-        lines.append("f " + " ".join("%d/%d/%d" % (V[v, 0] + 1, V[v, 1] + 1, V[v, 2] + 1) for v in t))
     p = tmp_path / "knot.obj"
-    p.write_text("\n".join(lines) + "\n")
+    write_obj(p, a)
     b = fa.Mesh.load(p).arrays()
     # per-triangle corners carry the same positions, texcoords and normals
     def corners(x):
