@@ -10,11 +10,14 @@
 // holding the same half values (tinyexr's float -> half rounding, below).
 #include <zlib.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fornos_export.h"
@@ -74,6 +77,70 @@ void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<ui
     be32(out, (uint32_t)crc32(0L, out.data() + at, (uInt)(data.size() + 4)));
 }
 
+// PNG image data as one zlib stream, built in parallel: the rows are split
+// into segments of about 1 MiB, each worker filters its rows (PNG filter 1,
+// "Sub": byte minus the byte 3 to the left, mod 256) and deflates them as a
+// raw deflate run of its own ending on a byte-aligned sync-flush block (the
+// last segment ends the stream).  A run never refers back into another
+// segment, so the concatenation is one valid deflate stream; its Adler-32
+// is combined from the segments' (adler32_combine).
+bool png_idat(const uint8_t* rgb, uint32_t width, uint32_t height, int level, std::vector<uint8_t>& z)
+{
+    const size_t stride = 3 * (size_t)width, line = stride + 1;
+    const uint32_t rows_per = (uint32_t)std::max<size_t>(1, ((size_t)1 << 20) / line);
+    const uint32_t nseg = (height + rows_per - 1) / rows_per;
+    std::vector<std::vector<uint8_t>> part(nseg);
+    std::vector<uLong> adl(nseg);
+    std::vector<size_t> len(nseg);
+    std::atomic<uint32_t> next{0};
+    std::atomic<bool> ok{true};
+    auto work = [&]() {
+        std::vector<uint8_t> raw;
+        for (uint32_t i; (i = next.fetch_add(1)) < nseg && ok.load();) {
+            const uint32_t y0 = i * rows_per, y1 = std::min(height, y0 + rows_per);
+            raw.resize((size_t)(y1 - y0) * line);
+            for (uint32_t y = y0; y < y1; ++y) {
+                const uint8_t* r = rgb + (size_t)y * stride;
+                uint8_t* o = raw.data() + (size_t)(y - y0) * line;
+                o[0] = 1;
+                for (size_t k = 0; k < stride; ++k) o[1 + k] = (uint8_t)(r[k] - (k >= 3 ? r[k - 3] : 0));
+            }
+            len[i] = raw.size();
+            adl[i] = adler32(adler32(0L, Z_NULL, 0), raw.data(), (uInt)raw.size());
+            z_stream zs{};
+            if (deflateInit2(&zs, level, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) {
+                ok = false;
+                return;
+            }
+            part[i].resize(deflateBound(&zs, (uLong)raw.size()) + 64);
+            zs.next_in = raw.data();
+            zs.avail_in = (uInt)raw.size();
+            zs.next_out = part[i].data();
+            zs.avail_out = (uInt)part[i].size();
+            const bool last = i + 1 == nseg;
+            const int r = deflate(&zs, last ? Z_FINISH : Z_SYNC_FLUSH);
+            if ((last ? r != Z_STREAM_END : r != Z_OK) || zs.avail_in != 0) ok = false;
+            part[i].resize(part[i].size() - zs.avail_out);
+            deflateEnd(&zs);
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nthreads = std::min<unsigned>(std::min<unsigned>(hw, 16u), nseg);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+    if (!ok) return false;
+    z.assign({0x78, 0x9c});  // deflate, 32 KiB window, no dictionary
+    uLong a = adl[0];
+    for (uint32_t i = 0; i < nseg; ++i) {
+        z.insert(z.end(), part[i].begin(), part[i].end());
+        if (i) a = adler32_combine(a, adl[i], (z_off_t)len[i]);
+    }
+    be32(z, (uint32_t)a);
+    return true;
+}
+
 // EXR attribute: name\0 type\0 size(int32) value
 void exr_attr(std::vector<uint8_t>& v, const char* name, const char* type, const std::vector<uint8_t>& val)
 {
@@ -125,18 +192,8 @@ void fornos_float_to_half(const float* in, uint16_t* out, size_t n)
 fornos_status fornos_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* rgb)
 {
     if (!path || !rgb || width == 0 || height == 0) return set_error(FORNOS_INVALID_ARG, "fornos_write_png: bad argument");
-    std::vector<uint8_t> raw;
-    raw.reserve((size_t)height * (3 * (size_t)width + 1));
-    for (uint32_t y = 0; y < height; ++y) {
-        raw.push_back(0);  // filter: none
-        const uint8_t* row = rgb + (size_t)y * width * 3;
-        raw.insert(raw.end(), row, row + (size_t)width * 3);
-    }
-    uLongf zlen = compressBound((uLong)raw.size());
-    std::vector<uint8_t> z(zlen);
-    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK)
-        return set_error(FORNOS_OOM, "fornos_write_png: deflate failed");
-    z.resize(zlen);
+    std::vector<uint8_t> z;
+    if (!png_idat(rgb, width, height, 6, z)) return set_error(FORNOS_OOM, "fornos_write_png: deflate failed");
     std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
     std::vector<uint8_t> ihdr;
     be32(ihdr, width);

@@ -108,3 +108,35 @@ def test_exr_round_trip(tmp_path):
         assert np.array_equal(img[name], expect)
     bad = (C.c_char_p * 2)(b"R", b"B")
     assert lib().fornos_write_exr(p.encode(), w, h, 2, bad, ptrs) != 0  # names must be sorted
+
+
+@pytest.mark.parametrize("w,h", [(1200, 1000), (1, 1), (700, 3)])
+def test_png_segments_form_one_zlib_stream(tmp_path, w, h):
+    # the writer deflates ~1 MiB row segments on several threads; the IDAT
+    # must still be one zlib stream with the right Adler-32 (zlib checks it)
+    # and Sub-filtered rows that decode to the input
+    import struct
+    import zlib
+    from PIL import Image
+    rng = np.random.default_rng(3)
+    yy, xx = np.mgrid[0:h, 0:w]
+    smooth = (128 + 100 * np.sin(xx / 37.0) * np.cos(yy / 53.0)).astype(np.uint8)
+    rgb = np.stack([smooth, rng.integers(0, 256, (h, w), dtype=np.uint8), smooth[::-1]], axis=2).copy()
+    p = str(tmp_path / "x.png")
+    assert lib().fornos_write_png(p.encode(), w, h, rgb.ctypes.data_as(C.c_void_p)) == 0
+    data = open(p, "rb").read()
+    pos, idat = 8, b""
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ = data[pos + 4:pos + 8]
+        if typ == b"IDAT":
+            idat += data[pos + 8:pos + 8 + n]
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 3 * w + 1)
+    assert np.all(raw[:, 0] == 1)
+    rows = raw[:, 1:].astype(np.int64)
+    out = rows.copy()
+    for k in range(3, 3 * w):
+        out[:, k] = (out[:, k] + out[:, k - 3]) & 0xFF
+    assert np.array_equal(out.astype(np.uint8).reshape(h, w, 3), rgb)
+    assert np.array_equal(np.asarray(Image.open(p).convert("RGB")), rgb)
