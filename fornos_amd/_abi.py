@@ -131,6 +131,7 @@ _SIGS = [
     ("fornos_solver_run_all", None, [H, C.POINTER(C.c_float)]),
     ("fornos_solver_progress", None, [H, C.POINTER(C.c_float)]),
     ("fornos_solver_reset", None, [H]),
+    ("fornos_solver_pair", None, [H, H]),
     ("fornos_solver_submit", None, [H, C.c_uint32]),
     ("fornos_solver_poll", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     ("fornos_solver_read", None, [H, P, C.c_size_t]),

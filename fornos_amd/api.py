@@ -387,6 +387,24 @@ class AmbientOcclusionSolver(Solver):
     def __init__(self, sampleCount=256, minDistance=0.01, maxDistance=10.0, seed=1234, table=None):
         super().__init__(sampleCount, minDistance, maxDistance, seed=seed, table=table)
 
+    def pair(self, bentNormals):
+        """Traces each ray once for this solver and a BentNormalsSolver with the
+        same rays (fornos_solver_pair); None unpairs.  Results are unchanged."""
+        check(lib().fornos_solver_pair(self._h, bentNormals.handle if bentNormals is not None else None))
+        self._partner = bentNormals  # the native pair holds no reference
+        return self
+
+
+def shares_rays(ao_params, bn_params):
+    """Whether AO and bent-normal solver keyword sets cast the same rays
+    (sample count, distances, sample seed / table), so they can be paired."""
+    f32 = lambda p, k, d: np.float32(p.get(k, d)).view(np.uint32)  # as the native side compares them
+    same = (int(ao_params.get("sampleCount", 256)) == int(bn_params.get("sampleCount", 256)) and
+            int(ao_params.get("seed", 1234)) == int(bn_params.get("seed", 1234)) and
+            f32(ao_params, "minDistance", 0.01) == f32(bn_params, "minDistance", 0.01) and
+            f32(ao_params, "maxDistance", 10.0) == f32(bn_params, "maxDistance", 10.0))
+    return same and ao_params.get("table") is None and bn_params.get("table") is None
+
 
 class BentNormalsSolver(Solver):
     kind, comps = _abi.SOLVER_BENT_NORMALS, 3

@@ -23,7 +23,7 @@ import time
 
 from . import _abi, configs, export
 from .api import (AmbientOcclusionSolver, BentNormalsSolver, HeightSolver, MeshMapping, NormalsSolver,
-                  PositionSolver, ThicknessSolver)
+                  PositionSolver, ThicknessSolver, shares_rays)
 from .runner import FornosParameters, FornosRunner, MeshMappingMethod, NormalImport
 
 SOLVERS = {"height": HeightSolver, "position": PositionSolver, "normals": NormalsSolver,
@@ -31,8 +31,9 @@ SOLVERS = {"height": HeightSolver, "position": PositionSolver, "normals": Normal
 EXT = {"position": "exr"}
 
 
-def bake(cfg, out_dir=None, ext="png", dilation=export.DEFAULT_DILATION, prep=None):
+def bake(cfg, out_dir=None, ext="png", dilation=export.DEFAULT_DILATION, prep=None, pair=True):
     """Runs the mapping and every map of `cfg`; exports when out_dir is set.
+    AO and bent normals casting the same rays share one traversal (pair).
     Returns per-map kernel ms and export seconds."""
     prep = prep or configs.prepare(cfg)
     cfg = prep["cfg"]
@@ -41,11 +42,13 @@ def bake(cfg, out_dir=None, ext="png", dilation=export.DEFAULT_DILATION, prep=No
     mm = MeshMapping().init_from_mesh(prep["low"], prep["dirs"], prep["width"], prep["height"], prep["scene"],
                                       cfg.get("cull", True), edge=prep["edge"])
     report["mapping_ms"] = mm.run_all()
-    solvers = {}
-    for name, kw in cfg["maps"].items():
-        s = SOLVERS[name](**kw).init(None, mm)
+    solvers = {name: SOLVERS[name](**kw).init(None, mm) for name, kw in cfg["maps"].items()}
+    maps = cfg["maps"]
+    if pair and "ao" in maps and "bentnormals" in maps and shares_rays(maps["ao"], maps["bentnormals"]):
+        solvers["ao"].pair(solvers["bentnormals"])  # one traversal for both maps
+        report["paired"] = ["ao", "bentnormals"]
+    for name, s in solvers.items():
         report[name + "_ms"] = s.run_all()
-        solvers[name] = s
     report["bake_s"] = time.perf_counter() - t0
     if out_dir:
         os.makedirs(out_dir, exist_ok=True)

@@ -20,7 +20,10 @@
 // culling tree.  Results therefore equal the reference's ray for ray.
 //
 // Only hits are written: AO adds 1 to its texel's counter, BN sets its
-// sample's bit, thickness stores t (the buffer starts at FLT_MAX).  k_reduce
+// sample's bit, thickness stores t (the buffer starts at FLT_MAX).  AO and
+// BN with the same rays (samples, minD, maxD) can share one traversal
+// (kAOBN): both count hits in reachable leaves, AO those at t < maxD and BN
+// any, so the AO search, which stops at its first hit, decides both.  k_reduce
 // then folds each texel's S results in sample order (step2 + tangentspace),
 // the same split as the reference's step1/step2 dispatches.
 #pragma once
@@ -29,7 +32,7 @@
 namespace fornos {
 namespace {
 
-enum { kAO = 0, kBN = 1, kThick = 2 };
+enum { kAO = 0, kBN = 1, kThick = 2, kAOBN = 3 };  // kAOBN: AO and bent normals from one traversal
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
 // counting run: box tests, triangle tests, rays, stack overflows, then per wave:
 // walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes
@@ -61,6 +64,7 @@ struct Lane {
     uint32_t node;      // binary trees: next node
     float best;         // thickness: closest accepted hit
     int mode;
+    bool bn;            // kAOBN: a qualifying hit at or beyond maxD was found (bent normals only)
     bool active;
 };
 
@@ -71,6 +75,9 @@ __device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
         if (hit) atomicAdd(&A.cnt[A.t0 + L.q], 1u);
     } else if (KIND == kBN) {
         if (hit) atomicOr(&A.mask[(size_t)L.q * A.mwords + (L.s >> 5)], 1u << (L.s & 31));
+    } else if (KIND == kAOBN) {
+        if (hit) atomicAdd(&A.cnt[A.t0 + L.q], 1u);
+        if (hit || L.bn) atomicOr(&A.mask[(size_t)L.q * A.mwords + (L.s >> 5)], 1u << (L.s & 31));
     } else {
         if (L.best != FLT_MAX) A.tval[(size_t)L.q * A.S + L.s] = L.best;
     }
@@ -79,13 +86,16 @@ __device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
 
 // Triangles [first, first + count) of a reference leaf whose node is `leaf`,
 // under the hit rules above.  Returns 1 when the ray is decided (AO/BN hit),
-// 2 when it must be redone on the reference tree, 0 otherwise.
+// 2 when it must be redone on the reference tree, 0 otherwise.  kAOBN: a
+// hit in a reachable leaf at t >= maxD decides bent normals only (L.bn) and
+// the search for an AO hit (t < maxD) goes on.
 template <int KIND, bool COUNT>
 __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint32_t count, uint32_t leaf,
                                 uint32_t& ct)
 {
     const uint32_t end = first + count;
     const float tri_hi = KIND == kAO ? A.maxd : INFINITY;
+    bool reach = false;  // kAOBN: the leaf's chain is known to pass
     for (uint32_t k0 = first; k0 < end; k0 += 4) {
         // the leaf's (n, a.n) records are contiguous: fetch four at once
         float4 q[4];
@@ -98,6 +108,15 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
             if (COUNT) ++ct;
             const float t = hemi_tri_q(L.ray.o, L.ray.d, q[j], k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi);
             if (t == FLT_MAX) continue;
+            if (KIND == kAOBN) {
+                if (!reach) {
+                    if (!chain_ok(L.ray, A.sc, leaf, A.maxd)) return 0;  // one chain per leaf
+                    reach = true;
+                }
+                if (t < A.maxd) return 1;
+                L.bn = true;
+                continue;
+            }
             if (KIND != kThick) return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 1 : 0;  // one chain per leaf
             if (chain_ok(L.ray, A.sc, leaf, fminf(A.maxd, t))) L.best = t;
             else return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 2 : 0;
@@ -188,11 +207,15 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
         if (w & kLeafFlag) {
             const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
             if (ref) {
-                const float tri_hi = KIND == kAO ? A.maxd : (KIND == kBN ? INFINITY : L.best);
+                const float tri_hi = KIND == kAO ? A.maxd : (KIND == kThick ? L.best : INFINITY);
                 for (uint32_t k = first; k < end; ++k) {
                     if (COUNT) ++ct;
                     const float t = hemi_tri(L.ray.o, L.ray.d, k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi);
                     if (t == FLT_MAX) continue;
+                    if (KIND == kAOBN && !(t < A.maxd)) {
+                        L.bn = true;
+                        continue;
+                    }
                     if (KIND != kThick) {
                         finish<KIND>(A, L, true);
                         return;
@@ -241,6 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
     L.best = FLT_MAX;
     L.mode = kWideTree;
     L.active = false;
+    L.bn = false;
     bool exhausted = false;
     uint32_t cn = 0, ct = 0, nr = 0, nov = 0;
     unsigned long long w_steps = 0, w_lanes = 0, l_steps = 0, l_lanes = 0, r_steps = 0, r_lanes = 0;
@@ -280,6 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                         L.slot = kNoChild;
                         L.node = 0;
                         L.best = FLT_MAX;
+                        L.bn = false;
                         L.active = true;
                         if (COUNT) ++nr;
                     }

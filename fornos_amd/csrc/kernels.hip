@@ -92,10 +92,15 @@ struct fornos_solver {
     bool timed = false;
     Marks marks;              // parts of the last fornos_solver_submit
     uint32_t mark_from = 0;   // texel its first part started at
+    uint64_t table_hash = 0;  // FNV-1a of the sample table (hemisphere solvers)
+    fornos_solver* partner = nullptr;  // AO: the BN solver its passes also compute
+    fornos_solver* owner = nullptr;    // BN: the AO solver it is paired with
     float rays_ms = 0, reduce_ms = 0;
     uint32_t launches = 0;
     ~fornos_solver()
     {
+        if (partner) partner->owner = nullptr;
+        if (owner) owner->partner = nullptr;
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
         dev_free(samples); dev_free(ray_samples); dev_free(out); dev_free(counts); dev_free(stats);
         dev_free(raybuf); dev_free(counter);
@@ -197,6 +202,89 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         HIP_TRY(hipGetLastError());
     }
     return FORNOS_OK;
+}
+
+// AO solver `s` and its BN partner over texels [t0, t1) with one traversal
+// (k_rays<kAOBN>), then both reductions.
+fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
+{
+    fornos_solver* b = s->partner;
+    fornos_meshmap* mm = s->mm;
+    fornos_scene* sc = mm->sc;
+    const uint32_t S = s->p.sample_count, mwords = (S + 31) / 32;
+    RayArgs A;
+    A.sc = sc->d;
+    A.coords = mm->coords;
+    A.tidx = mm->tidx;
+    A.samples = s->ray_samples;
+    A.cnt = s->counts;
+    A.mask = (uint32_t*)b->raybuf;
+    A.tval = nullptr;
+    A.counter = s->counter;
+    A.stats = s->stats;
+    A.S = S;
+    A.P = s->P;
+    A.mwords = mwords;
+    A.mind = s->p.min_distance;
+    A.maxd = s->p.max_distance;
+    A.refill = s->refill;
+    ReduceArgs Ra;
+    Ra.sc = sc->d;
+    Ra.coords = mm->coords;
+    Ra.tidx = mm->tidx;
+    Ra.samples = s->samples;
+    Ra.pixt = mm->pixt;
+    Ra.mask = nullptr;
+    Ra.tval = nullptr;
+    Ra.out = s->out;
+    Ra.counts = s->counts;
+    Ra.S = S;
+    Ra.P = s->P;
+    Ra.mwords = mwords;
+    Ra.maxd = s->p.max_distance;
+    Ra.tangent = 0;
+    ReduceArgs Rb = Ra;
+    Rb.samples = b->samples;
+    Rb.mask = A.mask;
+    Rb.out = b->out;
+    Rb.counts = b->counts;
+    Rb.tangent = b->p.tangent_space;
+    auto fill = [&](void* p, size_t n, uint32_t v) {
+        hipLaunchKernelGGL(k_fill, dim3(div_up(n, kBlock)), dim3(kBlock), 0, sc->stream, (uint32_t*)p, n, v);
+    };
+    for (uint32_t c0 = t0; c0 < t1; c0 += b->chunk_texels) {
+        const uint32_t nt = std::min(b->chunk_texels, t1 - c0);
+        A.t0 = Ra.t0 = Rb.t0 = c0;
+        A.ntex = Ra.ntex = Rb.ntex = nt;
+        const uint32_t blocks = std::min<uint32_t>(sc->cus * s->blocks_per_cu, div_up((uint64_t)nt * S, kBlock));
+        fill(s->counter, 1, 0);
+        fill(s->counts + c0, nt, 0);
+        fill(b->raybuf, (size_t)nt * mwords, 0);
+        hipEvent_t* ev = nullptr;
+        if (s->timed) {
+            s->events.resize(s->events.size() + 3, nullptr);
+            ev = &s->events[s->events.size() - 3];
+            for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+            HIP_TRY(hipEventRecord(ev[0], sc->stream));
+        }
+        launch_rays<kAOBN, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
+        if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
+        hipLaunchKernelGGL(k_reduce<kAO>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Ra);
+        hipLaunchKernelGGL(k_reduce<kBN>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Rb);
+        if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
+        HIP_TRY(hipGetLastError());
+    }
+    return FORNOS_OK;
+}
+
+// A paired AO solver computes its partner's results too while the two are
+// at the same point of their passes.
+bool fused(const fornos_solver* s) { return s->partner && s->partner->offset == s->offset; }
+
+// Work done by a pass of `s` also advances its partner.
+void follow(fornos_solver* s, bool was_fused)
+{
+    if (was_fused) s->partner->offset = s->offset;
 }
 
 fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool count)
@@ -690,6 +778,12 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         if (!table || n_perm == 0) return set_error(FORNOS_INVALID_ARG, "fornos_solver_create: sample table missing");
         s->P = n_perm;
         s->work_total = (uint64_t)mm->work_count * S;
+        {
+            uint64_t h = 1469598103934665603ull;
+            const uint8_t* b = (const uint8_t*)table;
+            for (size_t i = 0; i < sizeof(float) * 3 * (size_t)n_perm * S; ++i) h = (h ^ b[i]) * 1099511628211ull;
+            s->table_hash = h;
+        }
         std::vector<float4> t4((size_t)n_perm * S);
         for (size_t i = 0; i < t4.size(); ++i) t4[i] = make_float4(table[3 * i], table[3 * i + 1], table[3 * i + 2], 0.0f);
         if (fornos_status st = dev_alloc(&s->samples, t4.size())) return st;
@@ -760,8 +854,11 @@ fornos_status fornos_solver_run_step(fornos_solver* s, int* done)
             const uint64_t per = std::max<uint64_t>(1, kWorkPerStep / S) * S;
             const uint64_t work = std::min<uint64_t>(s->work_total - s->offset, per);
             const uint32_t t0 = (uint32_t)(s->offset / S), t1 = (uint32_t)std::min<uint64_t>((s->offset + work) / S, n);
-            if (fornos_status st = solver_launch(s, std::min(t0, n), t1, false)) return st;
+            const bool fz = fused(s);
+            if (fornos_status st = fz ? launch_pair(s, std::min(t0, n), t1) : solver_launch(s, std::min(t0, n), t1, false))
+                return st;
             s->offset += work;
+            follow(s, fz);
         } else {
             const uint64_t work = std::min<uint64_t>(s->work_total - s->offset, kWorkPerStep);
             const uint32_t t0 = (uint32_t)s->offset, t1 = (uint32_t)std::min<uint64_t>(s->offset + work, n);
@@ -784,10 +881,12 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     s->events.clear();
     s->timed = s->hemi();
-    const fornos_status st = solver_launch(s, t0, n, false);
+    const bool fz = fused(s);
+    const fornos_status st = fz ? launch_pair(s, t0, n) : solver_launch(s, t0, n, false);
     s->timed = false;
     if (st) return st;
     s->offset = s->work_total;
+    follow(s, fz);
     if (fornos_status st2 = tm.stop(ms)) return st2;
     s->rays_ms = s->reduce_ms = 0;
     s->launches = (uint32_t)(s->events.size() / 3);
@@ -826,13 +925,15 @@ fornos_status fornos_solver_submit(fornos_solver* s, uint32_t parts)
     const uint32_t S = s->hemi() ? s->p.sample_count : 1;
     const uint32_t t0 = (uint32_t)std::min<uint64_t>(s->offset / S, n);
     s->mark_from = t0;
+    const bool fz = fused(s);
     const uint32_t per = std::max<uint32_t>(1, div_up(n - t0, parts));
     for (uint32_t a = t0; a < n; a += per) {
         const uint32_t b = std::min(n, a + per);
-        if (fornos_status st = solver_launch(s, a, b, false)) return st;
+        if (fornos_status st = fz ? launch_pair(s, a, b) : solver_launch(s, a, b, false)) return st;
         if (fornos_status st = s->marks.add(s->mm->sc->stream, b)) return st;
     }
     s->offset = s->work_total;
+    follow(s, fz);
     return FORNOS_OK;
 }
 
@@ -843,11 +944,39 @@ fornos_status fornos_solver_poll(fornos_solver* s, float* progress, int* done)
                       done);
 }
 
+fornos_status fornos_solver_pair(fornos_solver* ao, fornos_solver* bn)
+{
+    if (!ao) return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: null handle");
+    if (ao->kind != FORNOS_SOLVER_AO) return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: first solver must be AO");
+    if (!bn) {
+        if (ao->partner) ao->partner->owner = nullptr;
+        ao->partner = nullptr;
+        return FORNOS_OK;
+    }
+    if (bn->kind != FORNOS_SOLVER_BENT_NORMALS)
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: second solver must be bent normals");
+    if (bn->mm != ao->mm) return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: solvers on different mesh mappings");
+    if (bn->p.sample_count != ao->p.sample_count || bn->P != ao->P || bn->table_hash != ao->table_hash)
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: different sample tables");
+    if (__builtin_bit_cast(uint32_t, bn->p.min_distance) != __builtin_bit_cast(uint32_t, ao->p.min_distance) ||
+        __builtin_bit_cast(uint32_t, bn->p.max_distance) != __builtin_bit_cast(uint32_t, ao->p.max_distance))
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: different min / max distances");
+    if ((bn->owner && bn->owner != ao) || (ao->partner && ao->partner != bn))
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_pair: a solver is already paired");
+    ao->partner = bn;
+    bn->owner = ao;
+    return FORNOS_OK;
+}
+
 fornos_status fornos_solver_reset(fornos_solver* s)
 {
     if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_reset: null handle");
     s->offset = 0;
     s->marks.clear();
+    if (s->partner) {  // a pair is reset together, so it runs fused again
+        s->partner->offset = 0;
+        s->partner->marks.clear();
+    }
     return FORNOS_OK;
 }
 

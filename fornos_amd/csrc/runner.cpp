@@ -7,13 +7,16 @@
 // the chosen method, the texel set, the hi-poly BVH, the mesh mapping, then
 // one task per enabled map pushed in the reference's order (thickness, bent
 // normals, AO, normals, positions, height) and the mesh mapping task last,
-// so that run(), which works on the back of the list, maps first.
+// so that run(), which works on the back of the list, maps first.  AO and
+// bent normals with the same rays are paired (fornos_solver_pair): the AO
+// task's pass computes both maps.
 //
 // run() never waits on the device: the first call on a task submits all of
 // its work (fornos_*_submit, kParts launches with a completion mark after
 // each), later calls poll the marks, and the call that finds the task done
 // runs its finish (the export each Task::finish performs) and drops it.
 #include <chrono>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <thread>
@@ -102,6 +105,7 @@ public:
         }
     }
     int kind() const override { return _kind; }
+    fornos_solver* solver() const { return _s; }
     bool exported = false;
     float range[2] = {0.0f, 0.0f};
 
@@ -288,6 +292,19 @@ fornos_status fornos_runner_start(fornos_runner* r, const fornos_parameters* p)
             return st;
         const int dilation = sp.kind == FORNOS_SOLVER_POSITION ? 0 : sh.tex_dilation;
         r->tasks.emplace_back(new SolverTask(sp.kind, s, has_text(m.output_path) ? m.output_path : "", dilation, sprm));
+    }
+    // AO and bent normals casting the same rays share one traversal: the AO
+    // task (run first) computes both, the BN task finds its work done
+    auto same_bits = [](float a, float b) { return std::memcmp(&a, &b, sizeof(float)) == 0; };
+    if (p->ao.enabled && p->bent_normals.enabled && p->ao.sample_count == p->bent_normals.sample_count &&
+        same_bits(p->ao.min_distance, p->bent_normals.min_distance) &&
+        same_bits(p->ao.max_distance, p->bent_normals.max_distance)) {
+        fornos_solver *ao = nullptr, *bn = nullptr;
+        for (auto& t : r->tasks) {
+            if (t->kind() == FORNOS_SOLVER_AO) ao = static_cast<SolverTask*>(t.get())->solver();
+            if (t->kind() == FORNOS_SOLVER_BENT_NORMALS) bn = static_cast<SolverTask*>(t.get())->solver();
+        }
+        if (fornos_status st = fornos_solver_pair(ao, bn)) return st;
     }
     r->tasks.emplace_back(new MappingTask(r->mm));
     return FORNOS_OK;

@@ -214,6 +214,17 @@ fornos_status fornos_solver_progress(const fornos_solver *s, float *out);
 /* Rewinds progress to 0 so the next run_step/run_all repeats the whole pass
  * (same inputs, same results). */
 fornos_status fornos_solver_reset(fornos_solver *s);
+/* Pairs an AO solver with a bent-normals solver that casts the same rays
+ * (same mesh mapping, sample table, sample count, min and max distance):
+ * while both stand at the same point of their passes, the AO solver's
+ * run_step / run_all / submit trace each ray once and produce both results
+ * (hits in reachable leaves at t < maxDistance count for AO, any for bent
+ * normals: Shaders/ao_step1.comp:157-165, bentnormals_step1.comp:170), and
+ * the BN solver's pass is then already done.  Results are identical to the
+ * two separate passes.  No reference counterpart (the reference runs the AO
+ * and BN tasks one after the other, Src/fornos.cpp:174-198).  bn = NULL
+ * unpairs.  Resetting the AO solver resets both. */
+fornos_status fornos_solver_pair(fornos_solver *ao, fornos_solver *bn);
 /* The same pair for a solver (runStep pacing of Src/solver_*.cpp). */
 fornos_status fornos_solver_submit(fornos_solver *s, uint32_t parts);
 fornos_status fornos_solver_poll(fornos_solver *s, float *progress, int *done);

@@ -216,3 +216,69 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
         assert np.array_equal(s.getResults().view(np.uint32), ref), shape
     with pytest.raises(fa.FornosError):
         s.set_launch(8, 16, 20)
+
+
+@pytest.mark.parametrize("maxd", [0.05, 0.3])
+def test_paired_ao_bn_equal_separate_passes(field, maxd):
+    # one traversal for AO and bent normals (kAOBN): AO counts hits at
+    # t < maxD in reachable leaves, BN any hit there; small maxD leaves many
+    # BN-only hits beyond maxD
+    S, tex = 64, field.prep["texels"]
+    ao0 = fa.AmbientOcclusionSolver(S, 0.01, maxd).init(tex, field.mm)
+    bn0 = fa.BentNormalsSolver(S, 0.01, maxd, True).init(tex, field.mm)
+    ao0.run_all()
+    bn0.run_all()
+    ao1 = fa.AmbientOcclusionSolver(S, 0.01, maxd).init(tex, field.mm)
+    bn1 = fa.BentNormalsSolver(S, 0.01, maxd, True).init(tex, field.mm)
+    ao1.pair(bn1)
+    ms = ao1.run_all()
+    assert bn1.run_all() < ms  # nothing left to trace
+    assert same_bits(ao0.getResults(), ao1.getResults()) and np.array_equal(ao0.counts(), ao1.counts())
+    assert same_bits(bn0.getResults(), bn1.getResults()) and np.array_equal(bn0.counts(), bn1.counts())
+    # the BN-only hits exist: BN occlusion exceeds AO occlusion somewhere
+    assert np.any(bn1.counts() > ao1.counts())
+    out, oc = orc.hemisphere(orc.KIND_BN, S, 0.01, maxd, bn1.table, field.coords, field.tidx, field.spos,
+                             field.snrm, field.nodes, 0, 96)
+    assert np.array_equal(bn1.counts()[:96], oc)
+
+
+def test_paired_steps_orders_and_rejections(field):
+    S, tex = 32, field.prep["texels"]
+    ref_ao = fa.AmbientOcclusionSolver(S, 0.01, 0.2).init(tex, field.mm)
+    ref_bn = fa.BentNormalsSolver(S, 0.01, 0.2, True).init(tex, field.mm)
+    ref_ao.run_all()
+    ref_bn.run_all()
+    a, b = ref_ao.getResults().view(np.uint32).copy(), ref_bn.getResults().view(np.uint32).copy()
+    ao = fa.AmbientOcclusionSolver(S, 0.01, 0.2).init(tex, field.mm)
+    bn = fa.BentNormalsSolver(S, 0.01, 0.2, True).init(tex, field.mm)
+    ao.pair(bn)
+    # run_step pacing: each AO step advances the BN solver as well
+    while not ao.runStep():
+        pass
+    assert bn.runStep()  # already done
+    assert np.array_equal(ao.getResults().view(np.uint32), a) and np.array_equal(bn.getResults().view(np.uint32), b)
+    # BN first: it runs alone, then AO runs alone (no stale pair state)
+    ao.reset()
+    bn.run_all()
+    ao.run_all()
+    assert np.array_equal(ao.getResults().view(np.uint32), a) and np.array_equal(bn.getResults().view(np.uint32), b)
+    # submit + poll pacing of the pair
+    import ctypes as C
+    from fornos_amd._abi import check, lib
+    ao.reset()
+    check(lib().fornos_solver_submit(ao.handle, 3))
+    prog, done = C.c_float(), C.c_int()
+    while True:
+        check(lib().fornos_solver_poll(bn.handle, C.byref(prog), C.byref(done)))
+        if done.value:
+            break
+    assert np.array_equal(ao.getResults().view(np.uint32), a) and np.array_equal(bn.getResults().view(np.uint32), b)
+    ao.pair(None)
+    # rays that differ cannot be paired
+    for other in (fa.BentNormalsSolver(S, 0.01, 0.3).init(tex, field.mm),
+                  fa.BentNormalsSolver(S + 1, 0.01, 0.2).init(tex, field.mm),
+                  fa.BentNormalsSolver(S, 0.01, 0.2, seed=99).init(tex, field.mm)):
+        with pytest.raises(fa.FornosError):
+            ao.pair(other)
+    with pytest.raises(fa.FornosError):  # the AO solver comes first
+        check(lib().fornos_solver_pair(bn.handle, ao.handle))
