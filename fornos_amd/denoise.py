@@ -175,9 +175,12 @@ class MeshJob:
         return mm
 
     def solvers(self, mm, S, seed=1234):
+        """AO and bent-normal solvers on the same rays, paired: one traversal
+        computes both (results identical to two passes)."""
         table = sample_table(S, seed=seed)[0]
         ao = AmbientOcclusionSolver(S, self.mind, self.maxd, table=table).init(None, mm)
         bn = BentNormalsSolver(S, self.mind, self.maxd, tangentSpace=False, table=table).init(None, mm)
+        ao.pair(bn)
         return ao, bn
 
     def gbuffers(self, mm):
