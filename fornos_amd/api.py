@@ -347,10 +347,11 @@ class Solver(_Handle):
     def count_work(self):
         a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
-        out = np.zeros(10, np.uint64)
-        check(lib().fornos_solver_work_counters(self._h, ptr(out), 10))
+        out = np.zeros(11, np.uint64)
+        check(lib().fornos_solver_work_counters(self._h, ptr(out), 11))
         w = {"node_tests": a.value, "tri_tests": b.value, "rays": c.value, "stack_overflows": d.value}
-        for k, v in zip(["walk_steps", "walk_lanes", "leaf_steps", "leaf_lanes", "refills", "refill_lanes"], out[4:]):
+        for k, v in zip(["walk_steps", "walk_lanes", "leaf_steps", "leaf_lanes", "refills", "refill_lanes",
+                         "reference_tree_rays"], out[4:]):
             w[k] = int(v)
         return w
 

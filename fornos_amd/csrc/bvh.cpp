@@ -11,6 +11,7 @@
 // through a scratch array) and builds disjoint subtrees on separate threads;
 // the result does not depend on the thread count.
 #include <algorithm>
+#include <functional>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -285,7 +286,8 @@ float fbits(uint32_t w)
 
 struct CullPrim {
     V3 lo, hi, c;
-    uint32_t ref;  // reference leaf node
+    uint32_t ref;       // reference leaf node
+    uint32_t mask = 0;  // a group of its triangles (bit j = first + j), 0: all
 };
 
 float true_area(V3 lo, V3 hi)
@@ -400,15 +402,85 @@ void emit_cull(const std::vector<CullNode>& pool, const std::vector<CullPrim>& p
 
 }  // namespace
 
-void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float pad, CullBVH& out)
+// Groups of at most 2 of a reference leaf's triangles (first, cnt), by
+// median splits of their centroids on the widest axis, each with its padded
+// box; appended to `parts`.
+void triangle_groups(const std::vector<float>& spos4, uint32_t first, uint32_t cnt, float pad, uint32_t leaf,
+                     std::vector<CullPrim>& parts)
 {
-    std::vector<CullPrim> prims;
+    auto vtx = [&](uint32_t j, int c) {
+        const size_t k = 3 * (size_t)(first + j) + c;
+        return mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
+    };
+    auto cen = [&](uint32_t j) { return (vtx(j, 0) + vtx(j, 1) + vtx(j, 2)) * (1.0f / 3.0f); };
+    std::vector<uint32_t> idx(cnt);
+    for (uint32_t j = 0; j < cnt; ++j) idx[j] = j;
+    std::function<void(uint32_t, uint32_t)> split = [&](uint32_t b, uint32_t e) {
+        if (e - b <= 2) {
+            CullPrim cp;
+            V3 l = mk(FLT_MAX, FLT_MAX, FLT_MAX), h = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            for (uint32_t q = b; q < e; ++q) {
+                for (int c = 0; c < 3; ++c) {
+                    l = hmin(l, vtx(idx[q], c));
+                    h = hmax(h, vtx(idx[q], c));
+                }
+                cp.mask |= 1u << idx[q];
+            }
+            cp.lo = l - mk(pad, pad, pad);
+            cp.hi = h + mk(pad, pad, pad);
+            cp.c = (cp.lo + cp.hi) * 0.5f;
+            cp.ref = leaf;
+            parts.push_back(cp);
+            return;
+        }
+        V3 cl = cen(idx[b]), ch = cl;
+        for (uint32_t q = b + 1; q < e; ++q) {
+            cl = hmin(cl, cen(idx[q]));
+            ch = hmax(ch, cen(idx[q]));
+        }
+        const V3 ex = ch - cl;
+        const int ax = (ex.x >= ex.y && ex.x >= ex.z) ? 0 : (ex.y >= ex.z ? 1 : 2);
+        const uint32_t mid = b + (e - b) / 2;
+        std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](uint32_t u, uint32_t v) {
+            const V3 a = cen(u), c = cen(v);
+            return ax == 0 ? a.x < c.x : (ax == 1 ? a.y < c.y : a.z < c.z);
+        });
+        split(b, mid);
+        split(mid, e);
+    };
+    split(0, cnt);
+}
+
+void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float pad, CullBVH& out, float split_cost,
+                    bool binary)
+{
+    std::vector<CullPrim> prims, parts;
     prims.reserve(ref.n_leaves);
     for (uint32_t i = 0; i < ref.n_nodes; ++i) {
         const uint32_t* w = &ref.nodes[8 * (size_t)i];
         if (!(w[7] & kLeafFlag)) continue;
         V3 lo = mk(fbits(w[0]), fbits(w[1]), fbits(w[2])), hi = mk(fbits(w[4]), fbits(w[5]), fbits(w[6]));
         const uint32_t first = w[3], cnt = w[7] & ~kLeafFlag;
+        if (split_cost > 0.0f && cnt >= 3 && cnt <= 32) {
+            // the groups replace the leaf when their expected triangle tests
+            // (each group's share of the leaf's surface area times its size)
+            // undercut the leaf's by more than split_cost
+            parts.clear();
+            triangle_groups(spos4, first, cnt, pad, i, parts);
+            V3 alo = parts[0].lo, ahi = parts[0].hi;
+            for (const CullPrim& cp : parts) {
+                alo = hmin(alo, cp.lo);
+                ahi = hmax(ahi, cp.hi);
+            }
+            const float area = true_area(alo, ahi);
+            float cost = split_cost;
+            for (const CullPrim& cp : parts)
+                cost += (area > 0 ? true_area(cp.lo, cp.hi) / area : 1.0f) * (float)__builtin_popcount(cp.mask);
+            if (cost < (float)cnt) {
+                prims.insert(prims.end(), parts.begin(), parts.end());
+                continue;
+            }
+        }
         for (uint32_t k = 3 * first; k < 3 * (first + cnt); ++k) {
             const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
             lo = hmin(lo, p);
@@ -426,10 +498,12 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
     CullBuilder cb{prims, {}};
     cb.pool.reserve(2 * prims.size());
     cb.build(0, (uint32_t)prims.size(), 0, out.depth);
-    // DFS pre-order with skip links, in the node format of the reference layout
-    out.nodes.reserve(8 * cb.pool.size());
-    out.ref_leaf.reserve(cb.pool.size());
-    emit_cull(cb.pool, prims, ref, 0, out);
+    if (binary) {
+        // DFS pre-order with skip links, in the node format of the reference layout
+        out.nodes.reserve(8 * cb.pool.size());
+        out.ref_leaf.reserve(cb.pool.size());
+        emit_cull(cb.pool, prims, ref, 0, out);
+    }
     // 4-wide collapse: open the largest-area internal child until 4 children
     struct Wide {
         const std::vector<CullNode>& pool;
@@ -469,7 +543,8 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
                     if (cn.child < 0) {
                         const uint32_t leaf = (uint32_t)(out.wide_leaves.size() / 4);
                         const uint32_t* rw = &ref.nodes[8 * (size_t)prims[cn.prim].ref];
-                        const uint32_t lw[4] = {rw[3], rw[7] & ~kLeafFlag, prims[cn.prim].ref, 0};
+                        const uint32_t lw[4] = {rw[3], rw[7] & ~kLeafFlag, prims[cn.prim].ref, prims[cn.prim].mask};
+                        if (prims[cn.prim].mask) ++out.n_groups;
                         out.wide_leaves.insert(out.wide_leaves.end(), lw, lw + 4);
                         c = kLeafFlag | leaf;
                     } else {
@@ -489,6 +564,67 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
     };
     Wide wd{cb.pool, prims, ref, out};
     wd.emit(0);
+    // Surface-area cost of the wide tree: each node's box tests and each
+    // leaf's triangle tests, weighted by the area of the box a ray has to
+    // meet first, over the root's area.
+    auto child_box = [&](const uint32_t* w, int k, V3& lo, V3& hi) {
+        float v[6];
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(&v[a], &w[4 * a + k], 4);
+            std::memcpy(&v[3 + a], &w[12 + 4 * a + k], 4);
+        }
+        lo = mk(v[0], v[1], v[2]);
+        hi = mk(v[3], v[4], v[5]);
+    };
+    double root = 0, cost = 0;
+    for (uint32_t n = 0; n < out.n_wide; ++n) {
+        const uint32_t* w = &out.wide[32 * (size_t)n];
+        V3 ulo = mk(FLT_MAX, FLT_MAX, FLT_MAX), uhi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        int nc = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (w[24 + k] == kNoChild) continue;
+            V3 lo, hi;
+            child_box(w, k, lo, hi);
+            ulo = hmin(ulo, lo);
+            uhi = hmax(uhi, hi);
+            ++nc;
+            if (w[24 + k] & kLeafFlag) {
+                const uint32_t* lf = &out.wide_leaves[4 * (size_t)(w[24 + k] & ~kLeafFlag)];
+                cost += (double)true_area(lo, hi) * (lf[3] ? (uint32_t)__builtin_popcount(lf[3]) : lf[1]);
+            }
+        }
+        const double a = true_area(ulo, uhi);
+        if (n == 0) root = a;
+        cost += a * kBoxTestCost * nc;
+    }
+    out.sah = root > 0 ? cost / root : 0;
+    // Predicted cost were every whole leaf of 3..32 triangles split, without
+    // building that tree: the leaf's triangle tests become one more level of
+    // box tests plus its triangle pairs' (in soup order) share of them.
+    double pred = cost;
+    for (const CullPrim& cp : prims) {
+        const uint32_t* rw = &ref.nodes[8 * (size_t)cp.ref];
+        const uint32_t first = rw[3], cnt = rw[7] & ~kLeafFlag;
+        if (cp.mask || cnt < 3 || cnt > 32) continue;
+        const double al = true_area(cp.lo, cp.hi);
+        double g = 0;
+        uint32_t ng = 0;
+        for (uint32_t j = 0; j < cnt; j += 2) {
+            const uint32_t m = std::min(2u, cnt - j);
+            V3 l = mk(FLT_MAX, FLT_MAX, FLT_MAX), h = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            for (uint32_t q = j; q < j + m; ++q)
+                for (int c = 0; c < 3; ++c) {
+                    const size_t k = 3 * (size_t)(first + q) + c;
+                    const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
+                    l = hmin(l, p);
+                    h = hmax(h, p);
+                }
+            g += (double)true_area(l - mk(pad, pad, pad), h + mk(pad, pad, pad)) * m;
+            ++ng;
+        }
+        pred += g - al * cnt + al * kBoxTestCost * std::min(4u, ng);
+    }
+    out.sah_split_pred = root > 0 ? pred / root : 0;
 }
 
 }  // namespace fornos

@@ -19,6 +19,13 @@
 // ray whose 4-wide stack would overflow is redone on the stackless binary
 // culling tree.  Results therefore equal the reference's ray for ray.
 //
+// AO walks a tree of its own when the scene gains from it: leaves split into
+// groups of at most 2 triangles with their own boxes.  A group whose box
+// the ray enters at or beyond maxD holds no hit below maxD, so pruning it is
+// exact for AO; BN and thickness count hits beyond maxD in any leaf the
+// reference reaches, so they keep whole leaves (whose boxes contain the
+// reference's leaf boxes).
+//
 // Only hits are written: AO adds 1 to its texel's counter, BN sets its
 // sample's bit, thickness stores t (the buffer starts at FLT_MAX).  AO and
 // BN with the same rays (samples, minD, maxD) can share one traversal
@@ -35,8 +42,9 @@ namespace {
 enum { kAO = 0, kBN = 1, kThick = 2, kAOBN = 3 };  // kAOBN: AO and bent normals from one traversal
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
 // counting run: box tests, triangle tests, rays, stack overflows, then per wave:
-// walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes
-constexpr uint32_t kStats = 10;
+// walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes;
+// then rays traced on the reference tree (slow directions and thickness redos)
+constexpr uint32_t kStats = 11;
 constexpr uint32_t kRefill = 16;  // default: refill a wave once this many lanes are idle
 constexpr uint32_t kQuorum = 8;   // leaf step once searching lanes <= parked lanes / kQuorum (0: wait for all)
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
@@ -84,18 +92,48 @@ __device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
     L.active = false;
 }
 
-// Triangles [first, first + count) of a reference leaf whose node is `leaf`,
-// under the hit rules above.  Returns 1 when the ray is decided (AO/BN hit),
-// 2 when it must be redone on the reference tree, 0 otherwise.  kAOBN: a
-// hit in a reachable leaf at t >= maxD decides bent normals only (L.bn) and
-// the search for an AO hit (t < maxD) goes on.
+// Triangles [first, first + count) of a reference leaf whose node is `leaf`
+// (all of them, or those in `mask`: bit j = triangle first + j), under the
+// hit rules above.  Returns 1 when the ray is decided (AO/BN hit), 2 when it
+// must be redone on the reference tree, 0 otherwise.  kAOBN: a hit in a
+// reachable leaf at t >= maxD decides bent normals only (L.bn) and the search
+// for an AO hit (t < maxD) goes on.
 template <int KIND, bool COUNT>
 __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint32_t count, uint32_t leaf,
-                                uint32_t& ct)
+                                uint32_t mask, uint32_t& ct)
 {
     const uint32_t end = first + count;
     const float tri_hi = KIND == kAO ? A.maxd : INFINITY;
     bool reach = false;  // kAOBN: the leaf's chain is known to pass
+    // a hit at t (FLT_MAX: none) -> -1 to go on, or the result
+    auto take = [&](float t) -> int {
+        if (t == FLT_MAX) return -1;
+        if (KIND == kAOBN) {
+            if (!reach) {
+                if (!chain_ok(L.ray, A.sc, leaf, A.maxd)) return 0;  // one chain per leaf
+                reach = true;
+            }
+            if (t < A.maxd) return 1;
+            L.bn = true;
+            return -1;
+        }
+        if (KIND != kThick) return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 1 : 0;  // one chain per leaf
+        if (chain_ok(L.ray, A.sc, leaf, fminf(A.maxd, t))) {
+            L.best = t;
+            return -1;
+        }
+        return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 2 : 0;
+    };
+    if (mask) {
+        while (mask) {
+            const uint32_t k = first + (uint32_t)__builtin_ctz(mask);
+            mask &= mask - 1;
+            if (COUNT) ++ct;
+            const int r = take(hemi_tri(L.ray.o, L.ray.d, k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi));
+            if (r >= 0) return r;
+        }
+        return 0;
+    }
     for (uint32_t k0 = first; k0 < end; k0 += 4) {
         // the leaf's (n, a.n) records are contiguous: fetch four at once
         float4 q[4];
@@ -106,20 +144,8 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
             const uint32_t k = k0 + j;
             if (k >= end) continue;
             if (COUNT) ++ct;
-            const float t = hemi_tri_q(L.ray.o, L.ray.d, q[j], k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi);
-            if (t == FLT_MAX) continue;
-            if (KIND == kAOBN) {
-                if (!reach) {
-                    if (!chain_ok(L.ray, A.sc, leaf, A.maxd)) return 0;  // one chain per leaf
-                    reach = true;
-                }
-                if (t < A.maxd) return 1;
-                L.bn = true;
-                continue;
-            }
-            if (KIND != kThick) return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 1 : 0;  // one chain per leaf
-            if (chain_ok(L.ray, A.sc, leaf, fminf(A.maxd, t))) L.best = t;
-            else return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 2 : 0;
+            const int r = take(hemi_tri_q(L.ray.o, L.ray.d, q[j], k, A.sc, A.mind, KIND == kThick ? L.best : tri_hi));
+            if (r >= 0) return r;
         }
     }
     return 0;
@@ -145,7 +171,7 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
         if (KIND == kThick && stk_t[L.sp * kBlock + tid] >= T) L.cur = kNoChild;
         if (L.cur == kNoChild || (L.cur & kLeafFlag)) return;
     }
-    const float4* nd = A.sc.wide + 8 * (size_t)L.cur;
+    const float4* nd = (KIND == kAO ? A.sc.wide_ao : A.sc.wide) + 8 * (size_t)L.cur;
     const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
     const uint4 ch = *(const uint4*)(nd + 6);
     const float4 lo[4] = {make_float4(lx.x, ly.x, lz.x, 0), make_float4(lx.y, ly.y, lz.y, 0),
@@ -193,7 +219,7 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
 // tested conservatively, hit rules above) or the reference tree (the
 // reference's own test and traversal, Shaders/ao_step1.comp:166-193).
 template <int KIND, bool COUNT>
-__device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint32_t& ct)
+__device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint32_t& ct, uint32_t& nref)
 {
     const bool ref = L.mode == kRefTree;
     const float4* nodes = ref ? A.sc.nodes : A.sc.cnodes;
@@ -223,12 +249,13 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
                     L.best = t;
                 }
             } else {
-                const int r = leaf_hits<KIND, COUNT>(A, L, first, w & ~kLeafFlag, A.sc.cref[L.node], ct);
+                const int r = leaf_hits<KIND, COUNT>(A, L, first, w & ~kLeafFlag, A.sc.cref[L.node], 0u, ct);
                 if (r == 1) {
                     finish<KIND>(A, L, true);
                     return;
                 }
                 if (r == 2) {
+                    if (COUNT) ++nref;
                     L.mode = kRefTree;
                     L.node = 0;
                     L.best = FLT_MAX;
@@ -266,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
     L.active = false;
     L.bn = false;
     bool exhausted = false;
-    uint32_t cn = 0, ct = 0, nr = 0, nov = 0;
+    uint32_t cn = 0, ct = 0, nr = 0, nov = 0, nref = 0;
     unsigned long long w_steps = 0, w_lanes = 0, l_steps = 0, l_lanes = 0, r_steps = 0, r_lanes = 0;
     for (;;) {
         // refill idle lanes, one atomic per wave
@@ -297,6 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                         L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
                                        fabsf(L.ray.d.z) >= kSlowDir);
                         L.mode = L.ray.slow ? kRefTree : kWideTree;
+                        if (COUNT && L.ray.slow) ++nref;
                         L.q = q;
                         L.s = __float_as_uint(rs.w);  // the sample's index in the table
                         L.cur = 0;
@@ -361,14 +389,15 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
         }
         if (L.active) {
             if (L.mode != kWideTree) {
-                binary_step<KIND, COUNT>(A, L, cn, ct);
+                binary_step<KIND, COUNT>(A, L, cn, ct, nref);
             } else if (L.slot != kNoChild) {  // (a lane still searching walks on)
-                const uint4 lf = A.sc.wleaf[L.slot & ~kLeafFlag];
+                const uint4 lf = (KIND == kAO ? A.sc.wleaf_ao : A.sc.wleaf)[L.slot & ~kLeafFlag];
                 L.slot = kNoChild;
-                const int r = leaf_hits<KIND, COUNT>(A, L, lf.x, lf.y, lf.z, ct);
+                const int r = leaf_hits<KIND, COUNT>(A, L, lf.x, lf.y, lf.z, lf.w, ct);
                 if (r == 1) {
                     finish<KIND>(A, L, true);
                 } else if (r == 2) {
+                    if (COUNT) ++nref;
                     L.mode = kRefTree;
                     L.node = 0;
                     L.best = FLT_MAX;
@@ -377,12 +406,13 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
         }
     }
     if (COUNT) {
-        unsigned long long a = cn, b = ct, c = nr, o = nov;
+        unsigned long long a = cn, b = ct, c = nr, o = nov, f = nref;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_down(a, off);
             b += __shfl_down(b, off);
             c += __shfl_down(c, off);
             o += __shfl_down(o, off);
+            f += __shfl_down(f, off);
         }
         if (lane == 0) {
             atomicAdd(&A.stats[4], w_steps);
@@ -392,6 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
             atomicAdd(&A.stats[8], r_steps);
             atomicAdd(&A.stats[9], r_lanes);
             atomicAdd(&A.stats[3], o);
+            atomicAdd(&A.stats[10], f);
             atomicAdd(&A.stats[0], a);
             atomicAdd(&A.stats[1], b);
             atomicAdd(&A.stats[2], c);

@@ -67,6 +67,11 @@ std::vector<uint32_t> reference_parents(const FlatBVH& B);
 // leaves are the reference leaves (same triangles), each box the union of
 // the reference leaf box and its triangles, padded by `pad`.  Same 8-word node
 // format as FlatBVH; ref_leaf[i] is the reference node of culling leaf i.
+// With split_cost > 0 a reference leaf of 3..32 triangles may instead enter
+// as several primitives, groups of at most 2 of its triangles with their own
+// padded boxes (kept when the surface-area estimate of its triangle tests
+// drops by more than split_cost tests); such a wide leaf carries the group's
+// triangle mask.
 struct CullBVH {
     std::vector<uint32_t> nodes;
     std::vector<uint32_t> ref_leaf;
@@ -74,11 +79,18 @@ struct CullBVH {
     // The same tree collapsed to 4 children per node, 32 words per node:
     // lo.x[4], lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4], child[4], pad[4].
     // child: node index, kLeafFlag | leaf index, or kNoChild.  Leaves: 4 words
-    // {first soup triangle, count, reference leaf node, 0}.
+    // {first soup triangle, count, reference leaf node, triangle mask (bit j =
+    // triangle first + j) or 0 for all of the leaf}.
     std::vector<uint32_t> wide, wide_leaves;
     uint32_t n_wide = 0;
+    uint32_t n_groups = 0;      // wide leaves that are triangle groups
+    double sah = 0;             // surface-area cost of the 4-wide tree, in triangle tests
+    double sah_split_pred = 0;  // the same predicted with every leaf of 3..32 triangles split
 };
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
-void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& soup_pos4, float pad, CullBVH& out);
+// binary: also emit the binary (skip-link) culling tree.
+void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& soup_pos4, float pad, CullBVH& out,
+                    float split_cost = 0.0f, bool binary = true);
+constexpr double kBoxTestCost = 0.6;  // one box test, in triangle tests (measured on the AO kernel)
 
 }  // namespace fornos

@@ -35,6 +35,12 @@ using namespace fornos;
 
 
 
+// AO's split-leaf tree: per-leaf split threshold (triangle tests), build it
+// when the predicted cost is below kSplitPredict of the plain tree's, keep it
+// when its own cost is below kSplitKeep of it.
+constexpr float kSplitCost = 1.0f;
+constexpr double kSplitPredict = 0.95, kSplitKeep = 0.9;
+
 struct fornos_scene {
     int device = 0;
     uint32_t cus = 256;  // compute units of the scene's device
@@ -44,13 +50,15 @@ struct fornos_scene {
     uint32_t *parent = nullptr, *cref = nullptr;
     float4* wide = nullptr;
     uint4* wleaf = nullptr;
+    float4* wide_ao = nullptr;  // AO's tree with split leaves (null: AO walks `wide`)
+    uint4* wleaf_ao = nullptr;
     FlatBVH bvh;
     fornos_scene_info info{};
     ~fornos_scene()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
-        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf);
+        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -381,11 +389,19 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     // largest coordinate magnitude (far above the predicate's rounding)
     float amax = 1.0f;
     for (const float4& q : spos) amax = std::max(amax, std::max(fabsf(q.x), std::max(fabsf(q.y), fabsf(q.z))));
-    CullBVH cull;
+    CullBVH cull, cull_ao;
     {
         std::vector<float> sp4((size_t)4 * spos.size());
         std::memcpy(sp4.data(), spos.data(), sizeof(float) * sp4.size());
         build_cull_bvh(B, sp4, amax * (1.0f / 65536.0f), cull);
+        // AO's own tree with leaves split into triangle groups, built when the
+        // surface-area estimate predicts a gain and kept when the built tree
+        // confirms it (elongated leaves, as on heightfields, gain most;
+        // compact ones gain nothing and only deepen the tree)
+        if (cull.sah_split_pred < kSplitPredict * cull.sah) {
+            build_cull_bvh(B, sp4, amax * (1.0f / 65536.0f), cull_ao, kSplitCost, false);
+            if (!(cull_ao.sah < kSplitKeep * cull.sah)) cull_ao = CullBVH();
+        }
     }
     const std::vector<uint32_t> parent = reference_parents(B);
     const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
@@ -407,6 +423,17 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     HIP_TRY(hipMemcpy(sc->wleaf, cull.wide_leaves.data(), sizeof(uint32_t) * cull.wide_leaves.size(), hipMemcpyHostToDevice));
     sc->d.wide = sc->wide;
     sc->d.wleaf = sc->wleaf;
+    sc->d.wide_ao = sc->wide;
+    sc->d.wleaf_ao = sc->wleaf;
+    if (cull_ao.n_wide) {
+        if (fornos_status st = dev_alloc(&sc->wide_ao, (size_t)8 * cull_ao.n_wide)) return st;
+        if (fornos_status st = dev_alloc(&sc->wleaf_ao, cull_ao.wide_leaves.size() / 4)) return st;
+        HIP_TRY(hipMemcpy(sc->wide_ao, cull_ao.wide.data(), sizeof(uint32_t) * cull_ao.wide.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sc->wleaf_ao, cull_ao.wide_leaves.data(), sizeof(uint32_t) * cull_ao.wide_leaves.size(),
+                          hipMemcpyHostToDevice));
+        sc->d.wide_ao = sc->wide_ao;
+        sc->d.wleaf_ao = sc->wleaf_ao;
+    }
     if (fornos_status st = dev_alloc(&sc->nodes, (size_t)2 * B.n_nodes)) return st;
     if (fornos_status st = dev_alloc(&sc->tri, tri.size())) return st;
     if (fornos_status st = dev_alloc(&sc->spos, spos.size())) return st;
@@ -429,7 +456,12 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     sc->info.build_seconds = build_s;
     sc->info.device_bytes = sizeof(float4) * (2 * (size_t)B.n_nodes + tri.size() + spos.size() + snrm.size() +
                                               2 * (size_t)cull.n_nodes) +
-                            sizeof(uint32_t) * (parent.size() + cull.ref_leaf.size());
+                            sizeof(uint32_t) * (parent.size() + cull.ref_leaf.size() + cull.wide.size() +
+                                                cull.wide_leaves.size() + cull_ao.wide.size() +
+                                                cull_ao.wide_leaves.size());
+    sc->info.ao_groups = cull_ao.n_groups;
+    sc->info.sah = cull.sah;
+    sc->info.sah_ao = cull_ao.n_wide ? cull_ao.sah : cull.sah;
     *out = sc.release();
     return FORNOS_OK;
 }

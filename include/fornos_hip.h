@@ -80,6 +80,11 @@ typedef struct fornos_scene_info {
     uint32_t depth;
     double build_seconds;   /* host BVH build + flatten */
     size_t device_bytes;    /* HBM held by the scene */
+    uint32_t ao_groups;     /* leaves of the AO tree that are triangle groups
+                               (0: AO walks the culling tree of the others) */
+    double sah;             /* surface-area cost of the 4-wide culling tree,
+                               in triangle tests per ray */
+    double sah_ao;          /* the same for the AO tree */
 } fornos_scene_info;
 
 typedef struct fornos_scene fornos_scene;
@@ -247,7 +252,9 @@ fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
 /* All counters of the last fornos_solver_count_work run: box tests, triangle
  * tests, rays, stack overflows; then, summed over waves: walk steps, lanes
  * walking, leaf steps, lanes in leaf steps, refills, lanes refilled (their
- * ratios are the SIMD lane utilisation of each phase).  n <= 10. */
+ * ratios are the SIMD lane utilisation of each phase); then rays traced on
+ * the reference tree (directions with a component below 2^-40, and
+ * thickness rays redone there).  n <= 11. */
 fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *counters, uint32_t n);
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
  * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal

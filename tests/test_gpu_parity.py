@@ -282,3 +282,21 @@ def test_paired_steps_orders_and_rejections(field):
             ao.pair(other)
     with pytest.raises(fa.FornosError):  # the AO solver comes first
         check(lib().fornos_solver_pair(bn.handle, ao.handle))
+
+
+def test_ao_tree_of_split_leaves(field):
+    # the heightfield's long reference leaves gain from AO's own tree of
+    # triangle groups; AO results through it equal the oracle's (the AO tests
+    # above run on this scene), while BN and thickness keep whole leaves
+    info = field.scene.info()
+    assert info["ao_groups"] > 0 and info["sah_ao"] < 0.9 * info["sah"]
+    s = fa.AmbientOcclusionSolver(48, 0.01, 0.2).init(field.prep["texels"], field.mm)
+    s.run_all()
+    w = s.count_work()
+    out, oc = orc.hemisphere(orc.KIND_AO, 48, 0.01, 0.2, s.table, field.coords, field.tidx, field.spos,
+                             field.snrm, field.nodes, 0, 256)
+    assert np.array_equal(s.counts()[:256], oc) and same_bits(s.getResults()[:256], out)
+    b = fa.BentNormalsSolver(48, 0.01, 0.2).init(field.prep["texels"], field.mm)
+    b.run_all()
+    wb = b.count_work()
+    assert w["tri_tests"] < wb["tri_tests"]  # the groups cut AO's triangle tests
