@@ -35,7 +35,7 @@ class SceneInfo(C.Structure):
     _fields_ = [("n_nodes", C.c_uint32), ("n_triangles", C.c_uint32), ("n_leaves", C.c_uint32),
                 ("max_leaf_tris", C.c_uint32), ("depth", C.c_uint32),
                 ("build_seconds", C.c_double), ("device_bytes", C.c_size_t), ("ao_groups", C.c_uint32),
-                ("sah", C.c_double), ("sah_ao", C.c_double)]
+                ("sah", C.c_double), ("sah_ao", C.c_double), ("leaf_group_nodes", C.c_uint32)]
 
 
 class TexelView(C.Structure):

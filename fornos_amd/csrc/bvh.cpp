@@ -627,6 +627,93 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
     out.sah_split_pred = root > 0 ? pred / root : 0;
 }
 
+// Group boxes inside the whole leaves of a culling tree: each wide leaf of
+// 3..32 triangles gets a node of up to 4 groups (two rounds of centroid
+// median splits), in the wide node format with a triangle mask in place of
+// each child, when the groups' surface-area share of its triangle tests plus
+// the four box tests (kBoxTestCost each) undercuts testing all of them.
+void add_leaf_groups(CullBVH& t, const std::vector<float>& spos4, float pad)
+{
+    auto vtx = [&](uint32_t tri, int c) {
+        const size_t k = 3 * (size_t)tri + c;
+        return mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
+    };
+    t.sub.clear();
+    t.n_sub = 0;
+    std::vector<uint32_t> idx;
+    for (size_t L = 0; L < t.wide_leaves.size(); L += 4) {
+        uint32_t* lw = &t.wide_leaves[L];
+        const uint32_t first = lw[0], cnt = lw[1];
+        lw[3] = 0;
+        if (cnt < 3 || cnt > 32) continue;
+        auto cen = [&](uint32_t j) { return (vtx(first + j, 0) + vtx(first + j, 1) + vtx(first + j, 2)) * (1.0f / 3.0f); };
+        idx.resize(cnt);
+        for (uint32_t j = 0; j < cnt; ++j) idx[j] = j;
+        auto halve = [&](uint32_t b, uint32_t e) {
+            V3 cl = cen(idx[b]), ch = cl;
+            for (uint32_t q = b + 1; q < e; ++q) {
+                cl = hmin(cl, cen(idx[q]));
+                ch = hmax(ch, cen(idx[q]));
+            }
+            const V3 ex = ch - cl;
+            const int ax = (ex.x >= ex.y && ex.x >= ex.z) ? 0 : (ex.y >= ex.z ? 1 : 2);
+            const uint32_t mid = b + (e - b) / 2;
+            std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](uint32_t u, uint32_t v) {
+                const V3 a = cen(u), c = cen(v);
+                return ax == 0 ? a.x < c.x : (ax == 1 ? a.y < c.y : a.z < c.z);
+            });
+            return mid;
+        };
+        const uint32_t m = halve(0, cnt);
+        const uint32_t cuts[5] = {0, m >= 2 ? halve(0, m) : m, m, cnt - m >= 2 ? halve(m, cnt) : cnt, cnt};
+        V3 glo[4], ghi[4];
+        uint32_t gmask[4] = {0, 0, 0, 0};
+        V3 alo = mk(FLT_MAX, FLT_MAX, FLT_MAX), ahi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        for (int g = 0; g < 4; ++g) {
+            glo[g] = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+            ghi[g] = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            for (uint32_t q = cuts[g]; q < cuts[g + 1]; ++q) {
+                for (int c = 0; c < 3; ++c) {
+                    glo[g] = hmin(glo[g], vtx(first + idx[q], c));
+                    ghi[g] = hmax(ghi[g], vtx(first + idx[q], c));
+                }
+                gmask[g] |= 1u << idx[q];
+            }
+            if (!gmask[g]) continue;
+            glo[g] = glo[g] - mk(pad, pad, pad);
+            ghi[g] = ghi[g] + mk(pad, pad, pad);
+            alo = hmin(alo, glo[g]);
+            ahi = hmax(ahi, ghi[g]);
+        }
+        const float area = true_area(alo, ahi);
+        float cost = 0;
+        int ng = 0;
+        for (int g = 0; g < 4; ++g)
+            if (gmask[g]) {
+                cost += (area > 0 ? true_area(glo[g], ghi[g]) / area : 1.0f) * (float)__builtin_popcount(gmask[g]);
+                ++ng;
+            }
+        cost += (float)(kBoxTestCost * ng);
+        if (!(cost < (float)cnt)) continue;
+        uint32_t w[32];
+        for (int g = 0; g < 4; ++g) {
+            float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            if (gmask[g]) {
+                lo[0] = glo[g].x; lo[1] = glo[g].y; lo[2] = glo[g].z;
+                hi[0] = ghi[g].x; hi[1] = ghi[g].y; hi[2] = ghi[g].z;
+            }
+            for (int a = 0; a < 3; ++a) {
+                std::memcpy(&w[4 * a + g], &lo[a], 4);
+                std::memcpy(&w[12 + 4 * a + g], &hi[a], 4);
+            }
+            w[24 + g] = gmask[g];
+            w[28 + g] = 0;
+        }
+        t.sub.insert(t.sub.end(), w, w + 32);
+        lw[3] = ++t.n_sub;
+    }
+}
+
 }  // namespace fornos
 
 extern "C" fornos_status fornos_bvh_reference_export(const fornos_mesh* m, uint32_t max_tris,

@@ -43,9 +43,10 @@ struct DevScene {
     const float4* __restrict__ cnodes;   // culling tree over the reference leaves
     const uint32_t* __restrict__ cref;   // reference leaf of each culling leaf
     const float4* __restrict__ wide;     // the culling tree collapsed to 4 children, 8 float4 per node
-    const uint4* __restrict__ wleaf;     // its leaves: first soup triangle, count, reference leaf, 0
+    const uint4* __restrict__ wleaf;     // its leaves: first soup triangle, count, reference leaf, group node + 1 or 0
     const float4* __restrict__ wide_ao;  // AO's 4-wide tree (leaves may be triangle groups), or `wide`
     const uint4* __restrict__ wleaf_ao;  // its leaves: first, count, reference leaf, triangle mask or 0
+    const float4* __restrict__ wsub;     // group nodes inside whole leaves of `wide` (wleaf.w = node + 1)
     uint32_t n_nodes, n_cnodes;
 };
 

@@ -93,7 +93,8 @@ __device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
 }
 
 // Triangles [first, first + count) of a reference leaf whose node is `leaf`
-// (all of them, or those in `mask`: bit j = triangle first + j), under the
+// (all of them, or, on AO's tree, those in `mask`: bit j = triangle first +
+// j; on the others' tree a non-zero `mask` is the leaf's group node + 1), under the
 // hit rules above.  Returns 1 when the ray is decided (AO/BN hit), 2 when it
 // must be redone on the reference tree, 0 otherwise.  kAOBN: a hit in a
 // reachable leaf at t >= maxD decides bent normals only (L.bn) and the search
@@ -124,6 +125,28 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
         }
         return chain_ok(L.ray, A.sc, leaf, A.maxd) ? 2 : 0;
     };
+    if (KIND != kAO && mask) {
+        // group boxes inside the whole leaf: a group whose padded box the ray
+        // enters no earlier than the pass's bound holds no hit that counts
+        // (thickness: its best hit; BN: hits at any distance count, so only
+        // groups the ray misses are skipped)
+        const float4* nd = A.sc.wsub + 8 * (size_t)(mask - 1);
+        const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+        const uint4 gm = *(const uint4*)(nd + 6);
+        const float T = KIND == kThick ? L.best : INFINITY;
+        const float4 lo[4] = {make_float4(lx.x, ly.x, lz.x, 0), make_float4(lx.y, ly.y, lz.y, 0),
+                              make_float4(lx.z, ly.z, lz.z, 0), make_float4(lx.w, ly.w, lz.w, 0)};
+        const float4 hi[4] = {make_float4(hx.x, hy.x, hz.x, 0), make_float4(hx.y, hy.y, hz.y, 0),
+                              make_float4(hx.z, hy.z, hz.z, 0), make_float4(hx.w, hy.w, hz.w, 0)};
+        const uint32_t gms[4] = {gm.x, gm.y, gm.z, gm.w};
+        mask = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            float al;
+            if (gms[g] && box_maybe(L.ray, lo[g], hi[g], T, al)) mask |= gms[g];
+        }
+        if (!mask) return 0;
+    }
     if (mask) {
         while (mask) {
             const uint32_t k = first + (uint32_t)__builtin_ctz(mask);

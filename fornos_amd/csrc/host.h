@@ -80,10 +80,16 @@ struct CullBVH {
     // lo.x[4], lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4], child[4], pad[4].
     // child: node index, kLeafFlag | leaf index, or kNoChild.  Leaves: 4 words
     // {first soup triangle, count, reference leaf node, triangle mask (bit j =
-    // triangle first + j) or 0 for all of the leaf}.
+    // triangle first + j) or 0 for all of the leaf (or, with group nodes, the
+    // leaf's group node + 1)}.
     std::vector<uint32_t> wide, wide_leaves;
     uint32_t n_wide = 0;
     uint32_t n_groups = 0;      // wide leaves that are triangle groups
+    // Group nodes inside whole leaves (add_leaf_groups): the wide node format
+    // with a triangle mask in place of each child; a whole wide leaf's word 3
+    // is then its group node + 1 (0: none).
+    std::vector<uint32_t> sub;
+    uint32_t n_sub = 0;
     double sah = 0;             // surface-area cost of the 4-wide tree, in triangle tests
     double sah_split_pred = 0;  // the same predicted with every leaf of 3..32 triangles split
 };
@@ -92,5 +98,6 @@ constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& soup_pos4, float pad, CullBVH& out,
                     float split_cost = 0.0f, bool binary = true);
 constexpr double kBoxTestCost = 0.6;  // one box test, in triangle tests (measured on the AO kernel)
+void add_leaf_groups(CullBVH& t, const std::vector<float>& soup_pos4, float pad);
 
 }  // namespace fornos

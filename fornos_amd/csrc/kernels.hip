@@ -52,13 +52,14 @@ struct fornos_scene {
     uint4* wleaf = nullptr;
     float4* wide_ao = nullptr;  // AO's tree with split leaves (null: AO walks `wide`)
     uint4* wleaf_ao = nullptr;
+    float4* wsub = nullptr;     // group nodes inside the whole leaves of `wide`
     FlatBVH bvh;
     fornos_scene_info info{};
     ~fornos_scene()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
-        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao);
+        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao); dev_free(wsub);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -402,6 +403,9 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
             build_cull_bvh(B, sp4, amax * (1.0f / 65536.0f), cull_ao, kSplitCost, false);
             if (!(cull_ao.sah < kSplitKeep * cull.sah)) cull_ao = CullBVH();
         }
+        // the other passes keep whole leaves; on such scenes group boxes
+        // inside them still spare triangle tests
+        if (cull_ao.n_wide) add_leaf_groups(cull, sp4, amax * (1.0f / 65536.0f));
     }
     const std::vector<uint32_t> parent = reference_parents(B);
     const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
@@ -425,6 +429,10 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     sc->d.wleaf = sc->wleaf;
     sc->d.wide_ao = sc->wide;
     sc->d.wleaf_ao = sc->wleaf;
+    if (fornos_status st = dev_alloc(&sc->wsub, (size_t)8 * std::max<uint32_t>(1, cull.n_sub))) return st;
+    if (cull.n_sub)
+        HIP_TRY(hipMemcpy(sc->wsub, cull.sub.data(), sizeof(uint32_t) * cull.sub.size(), hipMemcpyHostToDevice));
+    sc->d.wsub = sc->wsub;
     if (cull_ao.n_wide) {
         if (fornos_status st = dev_alloc(&sc->wide_ao, (size_t)8 * cull_ao.n_wide)) return st;
         if (fornos_status st = dev_alloc(&sc->wleaf_ao, cull_ao.wide_leaves.size() / 4)) return st;
@@ -458,8 +466,9 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
                                               2 * (size_t)cull.n_nodes) +
                             sizeof(uint32_t) * (parent.size() + cull.ref_leaf.size() + cull.wide.size() +
                                                 cull.wide_leaves.size() + cull_ao.wide.size() +
-                                                cull_ao.wide_leaves.size());
+                                                cull_ao.wide_leaves.size() + cull.sub.size());
     sc->info.ao_groups = cull_ao.n_groups;
+    sc->info.leaf_group_nodes = cull.n_sub;
     sc->info.sah = cull.sah;
     sc->info.sah_ao = cull_ao.n_wide ? cull_ao.sah : cull.sah;
     *out = sc.release();

@@ -85,6 +85,8 @@ typedef struct fornos_scene_info {
     double sah;             /* surface-area cost of the 4-wide culling tree,
                                in triangle tests per ray */
     double sah_ao;          /* the same for the AO tree */
+    uint32_t leaf_group_nodes;  /* whole leaves of the other passes' tree with
+                                   group boxes inside (built with the AO tree) */
 } fornos_scene_info;
 
 typedef struct fornos_scene fornos_scene;
