@@ -6,6 +6,7 @@ Names follow the reference classes: MeshMapping.init/runStep/progress
 native handles.
 """
 import ctypes as C
+import functools
 
 import numpy as np
 
@@ -184,6 +185,15 @@ def sample_table(sample_count, n_perm=SAMPLE_PERM_COUNT, seed=1234):
     return t, r
 
 
+@functools.lru_cache(maxsize=4)
+def _shared_table(sample_count, seed):
+    """The seeded table of a sample count, computed once for the solvers that
+    use it (read-only)."""
+    t, _ = sample_table(sample_count, SAMPLE_PERM_COUNT, seed)
+    t.setflags(write=False)
+    return t
+
+
 class Scene(_Handle):
     """Hi-poly mesh + BVH resident in HBM (BVH::createBinary + fillMeshData)."""
     _destroy = "fornos_scene_destroy"
@@ -301,7 +311,7 @@ class Solver(_Handle):
         tab, n_perm = None, 0
         if self.kind >= _abi.SOLVER_AO:
             if self.table is None:
-                self.table, _ = sample_table(self.params.sample_count, SAMPLE_PERM_COUNT, self.seed)
+                self.table = _shared_table(self.params.sample_count, self.seed)
             tab = np.ascontiguousarray(self.table, np.float32)
             n_perm = len(tab) // self.params.sample_count
         check(lib().fornos_solver_create(self.kind, C.byref(self.params), meshMapping.handle,

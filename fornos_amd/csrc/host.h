@@ -2,6 +2,8 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/fornos_host.h"
@@ -37,6 +39,27 @@ namespace fornos {
 
 fornos_status set_error(fornos_status st, const std::string& msg);
 fornos_status ok();
+
+// Runs fn(begin, end) over [0, n) split into contiguous chunks on up to 16
+// host threads (the calling thread takes the first chunk).
+template <typename Fn>
+void parallel_chunks(size_t n, Fn fn)
+{
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, n / 64));
+    if (nt <= 1) {
+        fn((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t b = t * per, e = std::min(n, b + per);
+        if (b < e) pool.emplace_back([=] { fn(b, e); });
+    }
+    fn((size_t)0, std::min(n, per));
+    for (std::thread& th : pool) th.join();
+}
 
 // mesh.cpp
 void vertex_normals(fornos_mesh& m);             // Src/mesh.cpp:650-678

@@ -835,7 +835,6 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         // sample's index, under which its result is stored.
         {
             std::vector<float4> r4(t4.size());
-            std::vector<std::pair<uint32_t, uint32_t>> key(S);
             auto spread = [](uint32_t v) {  // 16 bits -> even bit positions
                 v &= 0xffffu;
                 v = (v | (v << 8)) & 0x00ff00ffu;
@@ -848,18 +847,21 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
                 const float u = std::min(1.0f, std::max(0.0f, 0.5f * (x + 1.0f)));
                 return (uint32_t)(u * 65535.0f);
             };
-            for (uint32_t p = 0; p < n_perm; ++p) {
-                const float4* row = &t4[(size_t)p * S];
-                for (uint32_t i = 0; i < S; ++i) key[i] = {spread(quant(row[i].x)) | (spread(quant(row[i].y)) << 1), i};
-                std::stable_sort(key.begin(), key.end(),
-                                 [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-                                     return a.first < b.first;
-                                 });
-                for (uint32_t j = 0; j < S; ++j) {
-                    const float4 v = row[key[j].second];
-                    r4[(size_t)p * S + j] = make_float4(v.x, v.y, v.z, __builtin_bit_cast(float, key[j].second));
+            parallel_chunks(n_perm, [&](size_t p0, size_t p1) {
+                std::vector<std::pair<uint32_t, uint32_t>> key(S);
+                for (size_t p = p0; p < p1; ++p) {
+                    const float4* row = &t4[p * S];
+                    for (uint32_t i = 0; i < S; ++i) key[i] = {spread(quant(row[i].x)) | (spread(quant(row[i].y)) << 1), i};
+                    std::stable_sort(key.begin(), key.end(),
+                                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                                         return a.first < b.first;
+                                     });
+                    for (uint32_t j = 0; j < S; ++j) {
+                        const float4 v = row[key[j].second];
+                        r4[p * S + j] = make_float4(v.x, v.y, v.z, __builtin_bit_cast(float, key[j].second));
+                    }
                 }
-            }
+            });
             if (fornos_status st = dev_alloc(&s->ray_samples, r4.size())) return st;
             HIP_TRY(hipMemcpy(s->ray_samples, r4.data(), sizeof(float4) * r4.size(), hipMemcpyHostToDevice));
         }

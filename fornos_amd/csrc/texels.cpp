@@ -196,22 +196,31 @@ void fornos_texels_destroy(fornos_texels* t) { delete t; }
 fornos_status fornos_sample_table(uint32_t S, uint32_t P, uint32_t seed, float* out, uint32_t* rand_out)
 {
     if (!out || S == 0 || P == 0) return set_error(FORNOS_INVALID_ARG, "fornos_sample_table: bad argument");
+    // the permutations' random pairs in draw order, then the samples of each
+    // permutation (independent of one another) on several threads
     std::mt19937 re(seed);
+    std::vector<uint32_t> rnd(2 * (size_t)P);
     for (uint32_t j = 0; j < P; ++j) {
-        const uint32_t rx = (uint32_t)re(), ry = (uint32_t)re();
-        if (rand_out) { rand_out[2 * j] = rx; rand_out[2 * j + 1] = ry; }
-        for (uint32_t i = 0; i < S; ++i) {
-            const float x = (float)i / (float)S + (float)(rx & 0xffffu) / (float)(1 << 16);
-            const float y = (float)(reverse_bits(i) ^ ry) * 2.3283064365386963e-10f;
-            const float ux = x - (float)(long)x;
-            const float r = std::sqrt(ux);
-            const float phi = (float)(2.0 * 3.14159265) * y;
-            float* o = &out[3 * ((size_t)j * S + i)];
-            o[0] = r * std::cos(phi);
-            o[1] = r * std::sin(phi);
-            o[2] = std::sqrt(1.0f - ux);
-        }
+        rnd[2 * j] = (uint32_t)re();
+        rnd[2 * j + 1] = (uint32_t)re();
     }
+    if (rand_out) std::memcpy(rand_out, rnd.data(), sizeof(uint32_t) * rnd.size());
+    parallel_chunks(P, [&](size_t j0, size_t j1) {
+        for (uint32_t j = (uint32_t)j0; j < (uint32_t)j1; ++j) {
+            const uint32_t rx = rnd[2 * j], ry = rnd[2 * j + 1];
+            for (uint32_t i = 0; i < S; ++i) {
+                const float x = (float)i / (float)S + (float)(rx & 0xffffu) / (float)(1 << 16);
+                const float y = (float)(reverse_bits(i) ^ ry) * 2.3283064365386963e-10f;
+                const float ux = x - (float)(long)x;
+                const float r = std::sqrt(ux);
+                const float phi = (float)(2.0 * 3.14159265) * y;
+                float* o = &out[3 * ((size_t)j * S + i)];
+                o[0] = r * std::cos(phi);
+                o[1] = r * std::sin(phi);
+                o[2] = std::sqrt(1.0f - ux);
+            }
+        }
+    });
     return ok();
 }
 
