@@ -28,6 +28,7 @@
 #include "../../include/fornos_export.h"
 
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 
 #pragma clang fp contract(off)
 
@@ -55,9 +56,21 @@ struct fornos_scene {
     float4* wsub = nullptr;     // group nodes inside the whole leaves of `wide`
     FlatBVH bvh;
     fornos_scene_info info{};
+    // image export: its own stream (ordered after this scene's work when an
+    // export starts, so the next pass can run meanwhile) and a device
+    // workspace kept between exports; one export at a time
+    std::mutex xmu;
+    hipStream_t xstream = nullptr;
+    hipEvent_t xevent = nullptr;
+    uint8_t* xbuf = nullptr;
+    size_t xcap = 0;
     ~fornos_scene()
     {
+        if (xstream) (void)hipStreamSynchronize(xstream);
         if (stream) (void)hipStreamSynchronize(stream);
+        dev_free(xbuf);
+        if (xevent) (void)hipEventDestroy(xevent);
+        if (xstream) (void)hipStreamDestroy(xstream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
         dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao); dev_free(wsub);
         if (stream) (void)hipStreamDestroy(stream);
@@ -104,6 +117,7 @@ struct fornos_solver {
     uint64_t table_hash = 0;  // FNV-1a of the sample table (hemisphere solvers)
     fornos_solver* partner = nullptr;  // AO: the BN solver its passes also compute
     fornos_solver* owner = nullptr;    // BN: the AO solver it is paired with
+    hipEvent_t done_ev = nullptr;      // recorded after the solver's last queued work (exports wait on it)
     float rays_ms = 0, reduce_ms = 0;
     uint32_t launches = 0;
     ~fornos_solver()
@@ -114,6 +128,7 @@ struct fornos_solver {
         dev_free(samples); dev_free(ray_samples); dev_free(out); dev_free(counts); dev_free(stats);
         dev_free(raybuf); dev_free(counter);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (done_ev) (void)hipEventDestroy(done_ev);
     }
     bool hemi() const { return kind >= FORNOS_SOLVER_AO; }
     uint32_t comps() const
@@ -290,10 +305,22 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
 // at the same point of their passes.
 bool fused(const fornos_solver* s) { return s->partner && s->partner->offset == s->offset; }
 
-// Work done by a pass of `s` also advances its partner.
-void follow(fornos_solver* s, bool was_fused)
+// Marks the end of the work just queued for `s` (and for its partner when
+// the pass computed both), which exports of their results wait for.
+fornos_status mark_done(fornos_solver* s)
 {
-    if (was_fused) s->partner->offset = s->offset;
+    if (!s->done_ev) HIP_TRY(hipEventCreateWithFlags(&s->done_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(s->done_ev, s->mm->sc->stream));
+    return FORNOS_OK;
+}
+
+// Work done by a pass of `s` also advances its partner.
+fornos_status follow(fornos_solver* s, bool was_fused)
+{
+    if (fornos_status e = mark_done(s)) return e;
+    if (!was_fused) return FORNOS_OK;
+    s->partner->offset = s->offset;
+    return mark_done(s->partner);
 }
 
 fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool count)
@@ -901,12 +928,13 @@ fornos_status fornos_solver_run_step(fornos_solver* s, int* done)
             if (fornos_status st = fz ? launch_pair(s, std::min(t0, n), t1) : solver_launch(s, std::min(t0, n), t1, false))
                 return st;
             s->offset += work;
-            follow(s, fz);
+            if (fornos_status st = follow(s, fz)) return st;
         } else {
             const uint64_t work = std::min<uint64_t>(s->work_total - s->offset, kWorkPerStep);
             const uint32_t t0 = (uint32_t)s->offset, t1 = (uint32_t)std::min<uint64_t>(s->offset + work, n);
             if (fornos_status st = solver_launch(s, std::min(t0, n), t1, false)) return st;
             s->offset += work;
+            if (fornos_status st = mark_done(s)) return st;
         }
     }
     if (done) *done = s->offset >= s->work_total;
@@ -929,7 +957,7 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
     s->timed = false;
     if (st) return st;
     s->offset = s->work_total;
-    follow(s, fz);
+    if (fornos_status st2 = follow(s, fz)) return st2;
     if (fornos_status st2 = tm.stop(ms)) return st2;
     s->rays_ms = s->reduce_ms = 0;
     s->launches = (uint32_t)(s->events.size() / 3);
@@ -976,8 +1004,7 @@ fornos_status fornos_solver_submit(fornos_solver* s, uint32_t parts)
         if (fornos_status st = s->marks.add(s->mm->sc->stream, b)) return st;
     }
     s->offset = s->work_total;
-    follow(s, fz);
-    return FORNOS_OK;
+    return follow(s, fz);
 }
 
 fornos_status fornos_solver_poll(fornos_solver* s, float* progress, int* done)
@@ -1480,11 +1507,57 @@ fornos_status fornos_denoise_check_tiles(const float* input, const float* output
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    ~DevBuf() { dev_free(p); }
+// One export on its scene's export stream, holding the scene's export lock:
+// waits for the work queued on the scene stream so far, carves its device
+// buffers from the scene's workspace (grown when too small) and returns its
+// results through stream-ordered copies.
+struct Export {
+    fornos_scene* sc;
+    std::unique_lock<std::mutex> lock;
+    hipStream_t st = nullptr;
+    size_t used = 0;
+    explicit Export(const fornos_meshmap* mm) : sc(mm->sc), lock(mm->sc->xmu) {}
+    // after: the solver's last queued pass (or, without one, all work queued
+    // on the scene stream so far)
+    fornos_status begin(size_t bytes, const fornos_solver* after)
+    {
+        if (!sc->xstream) HIP_TRY(hipStreamCreateWithFlags(&sc->xstream, hipStreamNonBlocking));
+        if (!sc->xevent) HIP_TRY(hipEventCreateWithFlags(&sc->xevent, hipEventDisableTiming));
+        st = sc->xstream;
+        if (sc->xcap < bytes) {
+            HIP_TRY(hipStreamSynchronize(st));
+            dev_free(sc->xbuf);
+            sc->xbuf = nullptr;
+            sc->xcap = 0;
+            if (fornos_status e = dev_alloc(&sc->xbuf, bytes)) return e;
+            sc->xcap = bytes;
+        }
+        hipEvent_t wait = after->done_ev;
+        if (!wait) {
+            HIP_TRY(hipEventRecord(sc->xevent, sc->stream));
+            wait = sc->xevent;
+        }
+        HIP_TRY(hipStreamWaitEvent(st, wait, 0));
+        return FORNOS_OK;
+    }
+    template <typename T>
+    T* take(size_t count)
+    {
+        T* p = (T*)(sc->xbuf + used);
+        used += (sizeof(T) * std::max<size_t>(1, count) + 255) & ~(size_t)255;
+        return p;
+    }
+    fornos_status read(void* host, const void* dev, size_t bytes)
+    {
+        HIP_TRY(hipGetLastError());
+        if (bytes) HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return FORNOS_OK;
+    }
 };
+
+// workspace of any export of a map: 12 bytes per pixel at most, plus the range keys
+size_t export_bytes(const fornos_meshmap* mm) { return 12 * (size_t)mm->width * mm->height + 1024; }
 
 fornos_status export_check(const fornos_solver* s, uint32_t comps, const char* what)
 {
@@ -1498,22 +1571,20 @@ fornos_status export_check(const fornos_solver* s, uint32_t comps, const char* w
 }
 
 // getMinMax + the filter + computeScaleBias (Src/image.cpp:37-47, 72-80, 199-207)
-fornos_status export_range(const fornos_solver* s, float lo, float hi, int normalize, QuantArgs& A, float* minmax)
+fornos_status export_range(Export& X, const fornos_solver* s, float lo, float hi, int normalize, QuantArgs& A,
+                           float* minmax)
 {
     const fornos_meshmap* mm = s->mm;
-    hipStream_t st = mm->sc->stream;
+    hipStream_t st = X.st;
     float tmin = FLT_MAX, tmax = -FLT_MAX;
     if (mm->n) {
-        DevBuf<uint32_t> keys;
-        if (fornos_status e = dev_alloc(&keys.p, 2)) return e;
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys.p, (size_t)1, 0xffffffffu);
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys.p + 1, (size_t)1, 0u);
+        uint32_t* keys = X.take<uint32_t>(2);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys, (size_t)1, 0xffffffffu);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, keys + 1, (size_t)1, 0u);
         hipLaunchKernelGGL(k_minmax, dim3(std::min<uint32_t>(1024, div_up(mm->n, kBlock))), dim3(kBlock), 0, st,
-                           (const float*)s->out, mm->n, keys.p);
+                           (const float*)s->out, mm->n, keys);
         uint32_t k[2];
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(st));
-        HIP_TRY(hipMemcpy(k, keys.p, sizeof(k), hipMemcpyDeviceToHost));
+        if (fornos_status e = X.read(k, keys, sizeof(k))) return e;
         auto inv = [](uint32_t key) {
             const uint32_t u = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
             float f;
@@ -1549,36 +1620,31 @@ fornos_status export_range(const fornos_solver* s, float lo, float hi, int norma
 }
 
 // rgb (device, w*h*3) -> dilated copy back to the host
-fornos_status export_finish_rgb(const fornos_meshmap* mm, uint8_t* drgb, bool valid_from_rgb, int dilate,
+fornos_status export_finish_rgb(Export& X, const fornos_meshmap* mm, uint8_t* drgb, bool valid_from_rgb, int dilate,
                                 uint8_t* rgb)
 {
-    hipStream_t st = mm->sc->stream;
+    hipStream_t st = X.st;
     const size_t npix = (size_t)mm->width * mm->height;
-    DevBuf<uint8_t> valid, out;
     const uint8_t* res = drgb;
     if (dilate > 0 && npix) {
-        if (fornos_status e = dev_alloc(&valid.p, npix)) return e;
-        if (fornos_status e = dev_alloc(&out.p, npix * 3)) return e;
-        HIP_TRY(hipMemsetAsync(valid.p, 0, npix, st));
+        uint8_t* valid = X.take<uint8_t>(npix);
+        uint8_t* out = X.take<uint8_t>(npix * 3);
+        HIP_TRY(hipMemsetAsync(valid, 0, npix, st));
         if (valid_from_rgb) {
             if (mm->n)
                 hipLaunchKernelGGL(k_valid_rgb, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st,
                                    (const uint32_t*)mm->indices, mm->n, mm->width, mm->height, (const uint8_t*)drgb,
-                                   valid.p);
+                                   valid);
         } else if (mm->n) {
             hipLaunchKernelGGL(k_valid_texels, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st,
-                               (const uint32_t*)mm->indices, mm->n, valid.p);
+                               (const uint32_t*)mm->indices, mm->n, valid);
         }
-        HIP_TRY(hipMemcpyAsync(out.p, drgb, npix * 3, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(out, drgb, npix * 3, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_dilate, dim3(div_up(mm->width, kBlock), mm->height), dim3(kBlock), 0, st,
-                           (const uint8_t*)drgb, out.p, (const uint8_t*)valid.p, mm->width, mm->height,
-                           (uint32_t)dilate);
-        res = out.p;
+                           (const uint8_t*)drgb, out, (const uint8_t*)valid, mm->width, mm->height, (uint32_t)dilate);
+        res = out;
     }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
-    if (npix) HIP_TRY(hipMemcpy(rgb, res, npix * 3, hipMemcpyDeviceToHost));
-    return FORNOS_OK;
+    return X.read(rgb, res, npix * 3);
 }
 
 }  // namespace
@@ -1590,15 +1656,16 @@ fornos_status fornos_export_float_rgb8(const fornos_solver* s, float filter_lo, 
 {
     if (!rgb) return set_error(FORNOS_INVALID_ARG, "fornos_export_float_rgb8: null output");
     if (fornos_status e = export_check(s, 1, "fornos_export_float_rgb8")) return e;
-    QuantArgs A{};
-    if (fornos_status e = export_range(s, filter_lo, filter_hi, normalize, A, minmax)) return e;
     const fornos_meshmap* mm = s->mm;
+    Export X(mm);
+    if (fornos_status e = X.begin(export_bytes(mm), s)) return e;
+    QuantArgs A{};
+    if (fornos_status e = export_range(X, s, filter_lo, filter_hi, normalize, A, minmax)) return e;
     const size_t npix = (size_t)mm->width * mm->height;
-    DevBuf<uint8_t> d;
-    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
-    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3, mm->sc->stream));
-    if (mm->n) hipLaunchKernelGGL(k_quant_gray, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
-    return export_finish_rgb(mm, d.p, false, dilate, rgb);
+    uint8_t* d = X.take<uint8_t>(npix * 3);
+    HIP_TRY(hipMemsetAsync(d, 0, npix * 3, X.st));
+    if (mm->n) hipLaunchKernelGGL(k_quant_gray, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, X.st, A, d);
+    return export_finish_rgb(X, mm, d, false, dilate, rgb);
 }
 
 fornos_status fornos_export_float_f32(const fornos_solver* s, float filter_lo, float filter_hi, int normalize,
@@ -1606,18 +1673,16 @@ fornos_status fornos_export_float_f32(const fornos_solver* s, float filter_lo, f
 {
     if (!image) return set_error(FORNOS_INVALID_ARG, "fornos_export_float_f32: null output");
     if (fornos_status e = export_check(s, 1, "fornos_export_float_f32")) return e;
-    QuantArgs A{};
-    if (fornos_status e = export_range(s, filter_lo, filter_hi, normalize, A, minmax)) return e;
     const fornos_meshmap* mm = s->mm;
+    Export X(mm);
+    if (fornos_status e = X.begin(export_bytes(mm), s)) return e;
+    QuantArgs A{};
+    if (fornos_status e = export_range(X, s, filter_lo, filter_hi, normalize, A, minmax)) return e;
     const size_t npix = (size_t)mm->width * mm->height;
-    DevBuf<float> d;
-    if (fornos_status e = dev_alloc(&d.p, npix)) return e;
-    HIP_TRY(hipMemsetAsync(d.p, 0, npix * sizeof(float), mm->sc->stream));
-    if (mm->n) hipLaunchKernelGGL(k_float_plane, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
-    if (npix) HIP_TRY(hipMemcpy(image, d.p, npix * sizeof(float), hipMemcpyDeviceToHost));
-    return FORNOS_OK;
+    float* d = X.take<float>(npix);
+    HIP_TRY(hipMemsetAsync(d, 0, npix * sizeof(float), X.st));
+    if (mm->n) hipLaunchKernelGGL(k_float_plane, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, X.st, A, d);
+    return X.read(image, d, npix * sizeof(float));
 }
 
 fornos_status fornos_export_normal_rgb8(const fornos_solver* s, int dilate, uint8_t* rgb)
@@ -1625,6 +1690,8 @@ fornos_status fornos_export_normal_rgb8(const fornos_solver* s, int dilate, uint
     if (!rgb) return set_error(FORNOS_INVALID_ARG, "fornos_export_normal_rgb8: null output");
     if (fornos_status e = export_check(s, 3, "fornos_export_normal_rgb8")) return e;
     const fornos_meshmap* mm = s->mm;
+    Export X(mm);
+    if (fornos_status e = X.begin(export_bytes(mm), s)) return e;
     QuantArgs A{};
     A.v = s->out;
     A.indices = mm->indices;
@@ -1632,11 +1699,10 @@ fornos_status fornos_export_normal_rgb8(const fornos_solver* s, int dilate, uint
     A.w = mm->width;
     A.h = mm->height;
     const size_t npix = (size_t)mm->width * mm->height;
-    DevBuf<uint8_t> d;
-    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
-    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3, mm->sc->stream));
-    if (mm->n) hipLaunchKernelGGL(k_quant_normal, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
-    return export_finish_rgb(mm, d.p, true, dilate, rgb);
+    uint8_t* d = X.take<uint8_t>(npix * 3);
+    HIP_TRY(hipMemsetAsync(d, 0, npix * 3, X.st));
+    if (mm->n) hipLaunchKernelGGL(k_quant_normal, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, X.st, A, d);
+    return export_finish_rgb(X, mm, d, true, dilate, rgb);
 }
 
 fornos_status fornos_export_vector_f32(const fornos_solver* s, float* planes)
@@ -1644,6 +1710,8 @@ fornos_status fornos_export_vector_f32(const fornos_solver* s, float* planes)
     if (!planes) return set_error(FORNOS_INVALID_ARG, "fornos_export_vector_f32: null output");
     if (fornos_status e = export_check(s, 3, "fornos_export_vector_f32")) return e;
     const fornos_meshmap* mm = s->mm;
+    Export X(mm);
+    if (fornos_status e = X.begin(export_bytes(mm), s)) return e;
     QuantArgs A{};
     A.v = s->out;
     A.indices = mm->indices;
@@ -1651,14 +1719,10 @@ fornos_status fornos_export_vector_f32(const fornos_solver* s, float* planes)
     A.w = mm->width;
     A.h = mm->height;
     const size_t npix = (size_t)mm->width * mm->height;
-    DevBuf<float> d;
-    if (fornos_status e = dev_alloc(&d.p, npix * 3)) return e;
-    HIP_TRY(hipMemsetAsync(d.p, 0, npix * 3 * sizeof(float), mm->sc->stream));
-    if (mm->n) hipLaunchKernelGGL(k_vector_planes, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, mm->sc->stream, A, d.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
-    if (npix) HIP_TRY(hipMemcpy(planes, d.p, npix * 3 * sizeof(float), hipMemcpyDeviceToHost));
-    return FORNOS_OK;
+    float* d = X.take<float>(npix * 3);
+    HIP_TRY(hipMemsetAsync(d, 0, npix * 3 * sizeof(float), X.st));
+    if (mm->n) hipLaunchKernelGGL(k_vector_planes, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, X.st, A, d);
+    return X.read(planes, d, npix * 3 * sizeof(float));
 }
 
 fornos_status fornos_solver_map_size(const fornos_solver* s, uint32_t* width, uint32_t* height)

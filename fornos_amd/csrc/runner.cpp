@@ -14,9 +14,12 @@
 // run() never waits on the device: the first call on a task submits all of
 // its work (fornos_*_submit, kParts launches with a completion mark after
 // each), later calls poll the marks, and the call that finds the task done
-// runs its finish (the export each Task::finish performs) and drops it.
+// hands its finish (the export each Task::finish performs) to a host thread
+// and submits the next task, so exports overlap the following computation;
+// pending() holds until every finish has ended.
 #include <chrono>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <string>
 #include <thread>
@@ -139,9 +142,39 @@ struct fornos_runner {
     fornos_scene* scene = nullptr;
     fornos_meshmap* mm = nullptr;
     std::vector<std::unique_ptr<Task>> tasks;  // run from the back
+    // finished tasks whose finish (the export) runs on a host thread while
+    // the next task computes; with its status and message
+    struct Finishing {
+        std::unique_ptr<Task> task;
+        std::future<std::pair<fornos_status, std::string>> result;
+    };
+    std::vector<Finishing> finishing;
     std::vector<std::pair<int, std::pair<float, float>>> ranges;  // exported float maps
+    // Collects the finishes that have ended (wait: all of them); the first
+    // failure is returned with its message.
+    fornos_status collect(bool wait)
+    {
+        fornos_status first = FORNOS_OK;
+        std::string msg;
+        for (auto it = finishing.begin(); it != finishing.end();) {
+            if (!wait && it->result.wait_for(std::chrono::seconds(0)) != std::future_status::ready) {
+                ++it;
+                continue;
+            }
+            const auto res = it->result.get();
+            if (auto* s = dynamic_cast<SolverTask*>(it->task.get()))
+                if (s->exported && res.first == FORNOS_OK) ranges.push_back({s->kind(), {s->range[0], s->range[1]}});
+            if (res.first != FORNOS_OK && first == FORNOS_OK) {
+                first = res.first;
+                msg = res.second;
+            }
+            it = finishing.erase(it);
+        }
+        return first == FORNOS_OK ? FORNOS_OK : set_error(first, msg);
+    }
     void clear()
     {
+        (void)collect(true);  // exports still reading results end first
         tasks.clear();  // solvers before the mapping and the scene they use
         fornos_meshmap_destroy(mm);
         mm = nullptr;
@@ -310,12 +343,13 @@ fornos_status fornos_runner_start(fornos_runner* r, const fornos_parameters* p)
     return FORNOS_OK;
 }
 
-int fornos_runner_pending(const fornos_runner* r) { return r && !r->tasks.empty(); }
+int fornos_runner_pending(const fornos_runner* r) { return r && (!r->tasks.empty() || !r->finishing.empty()); }
 
 // Src/fornos.cpp:239-251
 fornos_status fornos_runner_run(fornos_runner* r)
 {
     if (!r) return set_error(FORNOS_INVALID_ARG, "fornos_runner_run: null handle");
+    if (fornos_status st = r->collect(false)) return st;
     if (r->tasks.empty()) return FORNOS_OK;
     Task& t = *r->tasks.back();
     if (!t.submitted) {
@@ -325,11 +359,19 @@ fornos_status fornos_runner_run(fornos_runner* r)
     int done = 0;
     if (fornos_status st = t.poll(&t.last_progress, &done)) return st;
     if (!done) return FORNOS_OK;
-    const fornos_status st = t.finish();
-    if (auto* s = dynamic_cast<SolverTask*>(&t))
-        if (s->exported && st == FORNOS_OK) r->ranges.push_back({s->kind(), {s->range[0], s->range[1]}});
+    // the task's finish runs on a host thread (exports wait on the device for
+    // the task's own work only), and the next task starts at once
+    std::unique_ptr<Task> task = std::move(r->tasks.back());
     r->tasks.pop_back();
-    return st;
+    Task* raw = task.get();
+    r->finishing.push_back({std::move(task), std::async(std::launch::async, [raw] {
+                                const fornos_status st = raw->finish();
+                                return std::make_pair(st, std::string(st ? fornos_last_error() : ""));
+                            })});
+    if (r->tasks.empty()) return FORNOS_OK;
+    Task& next = *r->tasks.back();
+    next.submitted = true;
+    return next.submit();
 }
 
 fornos_status fornos_runner_run_all(fornos_runner* r)
@@ -341,14 +383,15 @@ fornos_status fornos_runner_run_all(fornos_runner* r)
         if (fornos_status st = fornos_runner_run(r)) return st;
         if (r->tasks.size() == before && was_submitted) std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
-    return FORNOS_OK;
+    return r->collect(true);
 }
 
 fornos_status fornos_runner_current_task(fornos_runner* r, const char** name, float* progress)
 {
     if (!r) return set_error(FORNOS_INVALID_ARG, "fornos_runner_current_task: null handle");
     if (r->tasks.empty()) {
-        if (name) *name = nullptr;
+        // a finishing task (its export) is still the current one
+        if (name) *name = r->finishing.empty() ? nullptr : r->finishing.back().task->name();
         if (progress) *progress = 1.0f;
         return FORNOS_OK;
     }

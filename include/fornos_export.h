@@ -7,6 +7,12 @@
  * (w * h * 3 bytes), float buffers are planes of w * h floats.  The solver's
  * mesh mapping must carry map indices (a texel set built with indices or on
  * the device).  Output pointers are host memory.
+ *
+ * Exports run on the scene's export stream after the solver's last queued
+ * pass and return when their results are in host memory; other passes may
+ * run meanwhile (a host thread exporting one map while the next is baked).
+ * Exports of one scene take turns; a solver must not run again while it is
+ * being exported.
  */
 #ifndef FORNOS_EXPORT_H
 #define FORNOS_EXPORT_H
