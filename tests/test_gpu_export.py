@@ -86,3 +86,32 @@ def test_task_exports_write_decodable_files(bake, tmp_path):
             else:
                 ref, _ = ox.float_rgb8(s.getResults(), bake["idx"], w, w, 0, 0, True, 16)
             assert np.array_equal(back, ref), name
+
+
+def test_export_overlapping_a_running_pass_equals_a_quiet_one(bake):
+    # an export waits for its own solver's pass only: run a long AO pass on
+    # the same scene while exporting the finished maps from another thread,
+    # and the bytes equal the exports taken with nothing else running
+    from concurrent.futures import ThreadPoolExecutor
+    quiet = {k: fx.float_rgb8(bake[k], (0.0, 0.0), True, 16)[0] for k in ("ao", "th")}
+    quiet["nr"] = fx.normal_rgb8(bake["nr"], 16)
+    quiet["po"] = fx.vector_planes(bake["po"])
+    busy = fa.AmbientOcclusionSolver(1024, 0.01, 0.2).init(None, bake["mm"])
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        futs = {k: pool.submit(fx.float_rgb8, bake[k], (0.0, 0.0), True, 16) for k in ("ao", "th")}
+        futs["nr"] = pool.submit(fx.normal_rgb8, bake["nr"], 16)
+        futs["po"] = pool.submit(fx.vector_planes, bake["po"])
+        busy.run_all()
+        got = {k: f.result() for k, f in futs.items()}
+    for k in ("ao", "th"):
+        assert np.array_equal(got[k][0], quiet[k]), k
+    assert np.array_equal(got["nr"], quiet["nr"])
+    assert np.array_equal(got["po"].view(np.uint32), quiet["po"].view(np.uint32))
+    # and a solver exported right after its own pass was queued (no wait in
+    # between) gives its finished results
+    s = fa.AmbientOcclusionSolver(64, 0.01, 0.2).init(None, bake["mm"])
+    import ctypes as C
+    from fornos_amd._abi import check, lib
+    check(lib().fornos_solver_submit(s.handle, 4))
+    first = fx.float_rgb8(s, (0.0, 0.0), True, 0)[0]
+    assert np.array_equal(first, fx.float_rgb8(bake["ao"], (0.0, 0.0), True, 0)[0])
