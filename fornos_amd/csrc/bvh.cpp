@@ -11,6 +11,7 @@
 // through a scratch array) and builds disjoint subtrees on separate threads;
 // the result does not depend on the thread count.
 #include <algorithm>
+#include <array>
 #include <functional>
 #include <cfloat>
 #include <chrono>
@@ -302,6 +303,9 @@ struct CullNode {
     uint32_t prim = 0;         // leaf: primitive index
 };
 
+constexpr uint32_t kParallelPrims = 1u << 14;  // subtrees this large build their halves concurrently
+constexpr uint32_t kParallelDepth = 4;         // down to this depth (at most 16 threads)
+
 struct CullBuilder {
     std::vector<CullPrim>& prims;
     std::vector<CullNode> pool;
@@ -368,8 +372,29 @@ struct CullBuilder {
             mid = (uint32_t)(it - prims.begin());
             if (mid == b || mid == e) mid = b + (e - b) / 2;
         }
-        const int l = build(b, mid, depth + 1, max_depth);
-        const int r = build(mid, e, depth + 1, max_depth);
+        int l, r;
+        if (e - b >= kParallelPrims && depth < kParallelDepth) {
+            // the right half on another thread with a pool of its own (the
+            // halves touch disjoint ranges of `prims`), grafted after ours:
+            // the tree is the same as a sequential build's
+            CullBuilder rb{prims, {}};
+            uint32_t rdepth = 0;
+            auto right = std::async(std::launch::async, [&] { rb.build(mid, e, depth + 1, rdepth); });
+            l = build(b, mid, depth + 1, max_depth);
+            right.get();
+            r = (int)pool.size();
+            for (CullNode n : rb.pool) {
+                if (n.child >= 0) {
+                    n.child += r;
+                    n.right += r;
+                }
+                pool.push_back(n);
+            }
+            max_depth = std::max(max_depth, rdepth);
+        } else {
+            l = build(b, mid, depth + 1, max_depth);
+            r = build(mid, e, depth + 1, max_depth);
+        }
         pool[id].child = l;
         pool[id].right = r;
         return id;
@@ -454,45 +479,63 @@ void triangle_groups(const std::vector<float>& spos4, uint32_t first, uint32_t c
 void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float pad, CullBVH& out, float split_cost,
                     bool binary)
 {
-    std::vector<CullPrim> prims, parts;
-    prims.reserve(ref.n_leaves);
-    for (uint32_t i = 0; i < ref.n_nodes; ++i) {
-        const uint32_t* w = &ref.nodes[8 * (size_t)i];
-        if (!(w[7] & kLeafFlag)) continue;
-        V3 lo = mk(fbits(w[0]), fbits(w[1]), fbits(w[2])), hi = mk(fbits(w[4]), fbits(w[5]), fbits(w[6]));
-        const uint32_t first = w[3], cnt = w[7] & ~kLeafFlag;
-        if (split_cost > 0.0f && cnt >= 3 && cnt <= 32) {
-            // the groups replace the leaf when their expected triangle tests
-            // (each group's share of the leaf's surface area times its size)
-            // undercut the leaf's by more than split_cost
-            parts.clear();
-            triangle_groups(spos4, first, cnt, pad, i, parts);
-            V3 alo = parts[0].lo, ahi = parts[0].hi;
-            for (const CullPrim& cp : parts) {
-                alo = hmin(alo, cp.lo);
-                ahi = hmax(ahi, cp.hi);
-            }
-            const float area = true_area(alo, ahi);
-            float cost = split_cost;
-            for (const CullPrim& cp : parts)
-                cost += (area > 0 ? true_area(cp.lo, cp.hi) / area : 1.0f) * (float)__builtin_popcount(cp.mask);
-            if (cost < (float)cnt) {
-                prims.insert(prims.end(), parts.begin(), parts.end());
-                continue;
+    // primitives of each reference leaf, leaves in node order; computed in
+    // chunks of leaves on several threads and joined in the same order
+    std::vector<uint32_t> leaves;
+    leaves.reserve(ref.n_leaves);
+    for (uint32_t i = 0; i < ref.n_nodes; ++i)
+        if (ref.nodes[8 * (size_t)i + 7] & kLeafFlag) leaves.push_back(i);
+    const size_t nchunk = std::max<size_t>(1, std::min<size_t>(64, leaves.size() / 1024));
+    std::vector<std::vector<CullPrim>> chunk(nchunk);
+    parallel_chunks(nchunk, [&](size_t c0, size_t c1) {
+        std::vector<CullPrim> parts;
+        for (size_t c = c0; c < c1; ++c) {
+            const size_t l0 = leaves.size() * c / nchunk, l1 = leaves.size() * (c + 1) / nchunk;
+            std::vector<CullPrim>& out_c = chunk[c];
+            for (size_t li = l0; li < l1; ++li) {
+                const uint32_t i = leaves[li];
+                const uint32_t* w = &ref.nodes[8 * (size_t)i];
+                V3 lo = mk(fbits(w[0]), fbits(w[1]), fbits(w[2])), hi = mk(fbits(w[4]), fbits(w[5]), fbits(w[6]));
+                const uint32_t first = w[3], cnt = w[7] & ~kLeafFlag;
+                if (split_cost > 0.0f && cnt >= 3 && cnt <= 32) {
+                    // the groups replace the leaf when their expected triangle
+                    // tests (each group's share of the leaf's surface area
+                    // times its size) undercut the leaf's by more than split_cost
+                    parts.clear();
+                    triangle_groups(spos4, first, cnt, pad, i, parts);
+                    V3 alo = parts[0].lo, ahi = parts[0].hi;
+                    for (const CullPrim& cp : parts) {
+                        alo = hmin(alo, cp.lo);
+                        ahi = hmax(ahi, cp.hi);
+                    }
+                    const float area = true_area(alo, ahi);
+                    float cost = split_cost;
+                    for (const CullPrim& cp : parts)
+                        cost += (area > 0 ? true_area(cp.lo, cp.hi) / area : 1.0f) * (float)__builtin_popcount(cp.mask);
+                    if (cost < (float)cnt) {
+                        out_c.insert(out_c.end(), parts.begin(), parts.end());
+                        continue;
+                    }
+                }
+                for (uint32_t k = 3 * first; k < 3 * (first + cnt); ++k) {
+                    const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
+                    lo = hmin(lo, p);
+                    hi = hmax(hi, p);
+                }
+                CullPrim cp;
+                cp.lo = lo - mk(pad, pad, pad);
+                cp.hi = hi + mk(pad, pad, pad);
+                cp.c = (cp.lo + cp.hi) * 0.5f;
+                cp.ref = i;
+                out_c.push_back(cp);
             }
         }
-        for (uint32_t k = 3 * first; k < 3 * (first + cnt); ++k) {
-            const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
-            lo = hmin(lo, p);
-            hi = hmax(hi, p);
-        }
-        CullPrim cp;
-        cp.lo = lo - mk(pad, pad, pad);
-        cp.hi = hi + mk(pad, pad, pad);
-        cp.c = (cp.lo + cp.hi) * 0.5f;
-        cp.ref = i;
-        prims.push_back(cp);
-    }
+    }, 1);
+    std::vector<CullPrim> prims;
+    size_t total = 0;
+    for (const auto& c : chunk) total += c.size();
+    prims.reserve(total);
+    for (auto& c : chunk) prims.insert(prims.end(), c.begin(), c.end());
     out = CullBVH();
     if (prims.empty()) return;
     CullBuilder cb{prims, {}};
@@ -638,79 +681,91 @@ void add_leaf_groups(CullBVH& t, const std::vector<float>& spos4, float pad)
         const size_t k = 3 * (size_t)tri + c;
         return mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
     };
+    const size_t nleaf = t.wide_leaves.size() / 4;
+    std::vector<std::array<uint32_t, 32>> node(nleaf);  // each leaf's group node (kept when word 24..27 != 0)
+    std::vector<uint8_t> keep(nleaf, 0);
+    parallel_chunks(nleaf, [&](size_t L0, size_t L1) {
+        std::vector<uint32_t> idx;
+        for (size_t L = L0; L < L1; ++L) {
+            const uint32_t* lw = &t.wide_leaves[4 * L];
+            const uint32_t first = lw[0], cnt = lw[1];
+            if (cnt < 3 || cnt > 32) continue;
+            auto cen = [&](uint32_t j) {
+                return (vtx(first + j, 0) + vtx(first + j, 1) + vtx(first + j, 2)) * (1.0f / 3.0f);
+            };
+            idx.resize(cnt);
+            for (uint32_t j = 0; j < cnt; ++j) idx[j] = j;
+            auto halve = [&](uint32_t b, uint32_t e) {
+                V3 cl = cen(idx[b]), ch = cl;
+                for (uint32_t q = b + 1; q < e; ++q) {
+                    cl = hmin(cl, cen(idx[q]));
+                    ch = hmax(ch, cen(idx[q]));
+                }
+                const V3 ex = ch - cl;
+                const int ax = (ex.x >= ex.y && ex.x >= ex.z) ? 0 : (ex.y >= ex.z ? 1 : 2);
+                const uint32_t mid = b + (e - b) / 2;
+                std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](uint32_t u, uint32_t v) {
+                    const V3 a = cen(u), c = cen(v);
+                    return ax == 0 ? a.x < c.x : (ax == 1 ? a.y < c.y : a.z < c.z);
+                });
+                return mid;
+            };
+            const uint32_t m = halve(0, cnt);
+            const uint32_t cuts[5] = {0, m >= 2 ? halve(0, m) : m, m, cnt - m >= 2 ? halve(m, cnt) : cnt, cnt};
+            V3 glo[4], ghi[4];
+            uint32_t gmask[4] = {0, 0, 0, 0};
+            V3 alo = mk(FLT_MAX, FLT_MAX, FLT_MAX), ahi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            for (int g = 0; g < 4; ++g) {
+                glo[g] = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+                ghi[g] = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+                for (uint32_t q = cuts[g]; q < cuts[g + 1]; ++q) {
+                    for (int c = 0; c < 3; ++c) {
+                        glo[g] = hmin(glo[g], vtx(first + idx[q], c));
+                        ghi[g] = hmax(ghi[g], vtx(first + idx[q], c));
+                    }
+                    gmask[g] |= 1u << idx[q];
+                }
+                if (!gmask[g]) continue;
+                glo[g] = glo[g] - mk(pad, pad, pad);
+                ghi[g] = ghi[g] + mk(pad, pad, pad);
+                alo = hmin(alo, glo[g]);
+                ahi = hmax(ahi, ghi[g]);
+            }
+            const float area = true_area(alo, ahi);
+            float cost = 0;
+            int ng = 0;
+            for (int g = 0; g < 4; ++g)
+                if (gmask[g]) {
+                    cost += (area > 0 ? true_area(glo[g], ghi[g]) / area : 1.0f) * (float)__builtin_popcount(gmask[g]);
+                    ++ng;
+                }
+            cost += (float)(kBoxTestCost * ng);
+            if (!(cost < (float)cnt)) continue;
+            uint32_t* w = node[L].data();
+            for (int g = 0; g < 4; ++g) {
+                float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                if (gmask[g]) {
+                    lo[0] = glo[g].x; lo[1] = glo[g].y; lo[2] = glo[g].z;
+                    hi[0] = ghi[g].x; hi[1] = ghi[g].y; hi[2] = ghi[g].z;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    std::memcpy(&w[4 * a + g], &lo[a], 4);
+                    std::memcpy(&w[12 + 4 * a + g], &hi[a], 4);
+                }
+                w[24 + g] = gmask[g];
+                w[28 + g] = 0;
+            }
+            keep[L] = 1;
+        }
+    });
+    // group nodes in leaf order
     t.sub.clear();
     t.n_sub = 0;
-    std::vector<uint32_t> idx;
-    for (size_t L = 0; L < t.wide_leaves.size(); L += 4) {
-        uint32_t* lw = &t.wide_leaves[L];
-        const uint32_t first = lw[0], cnt = lw[1];
-        lw[3] = 0;
-        if (cnt < 3 || cnt > 32) continue;
-        auto cen = [&](uint32_t j) { return (vtx(first + j, 0) + vtx(first + j, 1) + vtx(first + j, 2)) * (1.0f / 3.0f); };
-        idx.resize(cnt);
-        for (uint32_t j = 0; j < cnt; ++j) idx[j] = j;
-        auto halve = [&](uint32_t b, uint32_t e) {
-            V3 cl = cen(idx[b]), ch = cl;
-            for (uint32_t q = b + 1; q < e; ++q) {
-                cl = hmin(cl, cen(idx[q]));
-                ch = hmax(ch, cen(idx[q]));
-            }
-            const V3 ex = ch - cl;
-            const int ax = (ex.x >= ex.y && ex.x >= ex.z) ? 0 : (ex.y >= ex.z ? 1 : 2);
-            const uint32_t mid = b + (e - b) / 2;
-            std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](uint32_t u, uint32_t v) {
-                const V3 a = cen(u), c = cen(v);
-                return ax == 0 ? a.x < c.x : (ax == 1 ? a.y < c.y : a.z < c.z);
-            });
-            return mid;
-        };
-        const uint32_t m = halve(0, cnt);
-        const uint32_t cuts[5] = {0, m >= 2 ? halve(0, m) : m, m, cnt - m >= 2 ? halve(m, cnt) : cnt, cnt};
-        V3 glo[4], ghi[4];
-        uint32_t gmask[4] = {0, 0, 0, 0};
-        V3 alo = mk(FLT_MAX, FLT_MAX, FLT_MAX), ahi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
-        for (int g = 0; g < 4; ++g) {
-            glo[g] = mk(FLT_MAX, FLT_MAX, FLT_MAX);
-            ghi[g] = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
-            for (uint32_t q = cuts[g]; q < cuts[g + 1]; ++q) {
-                for (int c = 0; c < 3; ++c) {
-                    glo[g] = hmin(glo[g], vtx(first + idx[q], c));
-                    ghi[g] = hmax(ghi[g], vtx(first + idx[q], c));
-                }
-                gmask[g] |= 1u << idx[q];
-            }
-            if (!gmask[g]) continue;
-            glo[g] = glo[g] - mk(pad, pad, pad);
-            ghi[g] = ghi[g] + mk(pad, pad, pad);
-            alo = hmin(alo, glo[g]);
-            ahi = hmax(ahi, ghi[g]);
-        }
-        const float area = true_area(alo, ahi);
-        float cost = 0;
-        int ng = 0;
-        for (int g = 0; g < 4; ++g)
-            if (gmask[g]) {
-                cost += (area > 0 ? true_area(glo[g], ghi[g]) / area : 1.0f) * (float)__builtin_popcount(gmask[g]);
-                ++ng;
-            }
-        cost += (float)(kBoxTestCost * ng);
-        if (!(cost < (float)cnt)) continue;
-        uint32_t w[32];
-        for (int g = 0; g < 4; ++g) {
-            float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            if (gmask[g]) {
-                lo[0] = glo[g].x; lo[1] = glo[g].y; lo[2] = glo[g].z;
-                hi[0] = ghi[g].x; hi[1] = ghi[g].y; hi[2] = ghi[g].z;
-            }
-            for (int a = 0; a < 3; ++a) {
-                std::memcpy(&w[4 * a + g], &lo[a], 4);
-                std::memcpy(&w[12 + 4 * a + g], &hi[a], 4);
-            }
-            w[24 + g] = gmask[g];
-            w[28 + g] = 0;
-        }
-        t.sub.insert(t.sub.end(), w, w + 32);
-        lw[3] = ++t.n_sub;
+    for (size_t L = 0; L < nleaf; ++L) {
+        t.wide_leaves[4 * L + 3] = 0;
+        if (!keep[L]) continue;
+        t.sub.insert(t.sub.end(), node[L].begin(), node[L].end());
+        t.wide_leaves[4 * L + 3] = ++t.n_sub;
     }
 }
 

@@ -40,13 +40,13 @@ namespace fornos {
 fornos_status set_error(fornos_status st, const std::string& msg);
 fornos_status ok();
 
-// Runs fn(begin, end) over [0, n) split into contiguous chunks on up to 16
-// host threads (the calling thread takes the first chunk).
+// Runs fn(begin, end) over [0, n) split into contiguous chunks of at least
+// `grain` items on up to 16 host threads (the calling thread takes the first).
 template <typename Fn>
-void parallel_chunks(size_t n, Fn fn)
+void parallel_chunks(size_t n, Fn fn, size_t grain = 64)
 {
     const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, n / 64));
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, n / std::max<size_t>(1, grain)));
     if (nt <= 1) {
         fn((size_t)0, n);
         return;
