@@ -18,7 +18,8 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = "heightfield2048"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes (see DESIGN.md "Measurement"):
+# Algorithmic bytes of an AO pass: per texel, per ray, per box test and per
+# triangle test (the test counts come from a counting run of the same kernel)
 TEXEL_BYTES = 16 + 4 + 96 + 4  # coords + tidx in, hit-triangle positions+normals, AO out
 RAY_BYTES = 16                 # one sample direction
 NODE_BYTES = 32                # one node test
