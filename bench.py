@@ -26,9 +26,10 @@ NODE_BYTES = 32                # one node test
 TRI_BYTES = 64                 # one triangle test (predicate record)
 
 
-def load_traffic():
-    """HBM bytes per AO launch from the committed rocprofv3 --pmc summary, if any."""
-    p = os.path.join(ROOT, "profiles", "ao_heightfield2048_pmc.json")
+def load_traffic(workload):
+    """HBM bytes per AO launch of this workload from the committed rocprofv3
+    --pmc summary, if there is one (profiles/ao_<workload>_pmc.json)."""
+    p = os.path.join(ROOT, "profiles", "ao_%s_pmc.json" % workload)
     if not os.path.exists(p):
         return None
     try:
@@ -174,7 +175,7 @@ def main():
                 + work["tri_tests"] * TRI_BYTES)
     launch_ms = float(np.mean(rays_ms)) / launches
     achieved = alg_pass / launches / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_rays<AO>", "launches_per_step": launches, "launch_ms": launch_ms,
