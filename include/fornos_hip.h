@@ -261,8 +261,8 @@ fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *coun
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
  * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
  * stack entries per lane (16, 24 or 32; a ray that would overflow it is
- * redone on a stackless tree, so results do not depend on it).  No reference
- * counterpart (the shaders' dispatch shape is fixed). */
+ * redone on a stackless tree, so results do not depend on it); default 8, 32,
+ * 16.  No reference counterpart (the shaders' dispatch shape is fixed). */
 fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu, uint32_t refill,
                                        uint32_t stack);
 fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
