@@ -12,7 +12,7 @@
 // task's pass computes both maps.
 //
 // run() never waits on the device: the first call on a task submits all of
-// its work (fornos_*_submit, kParts launches with a completion mark after
+// its work (fornos_*_submit, a few launches with a completion mark after
 // each), later calls poll the marks, and the call that finds the task done
 // hands its finish (the export each Task::finish performs) to a host thread
 // and submits the next task, so exports overlap the following computation;
@@ -32,7 +32,12 @@
 namespace fornos {
 namespace {
 
-constexpr uint32_t kParts = 32;          // submission parts per task (progress granularity)
+// Submission parts of a task: one per 2^27 rays (or texels), 1..32.  Each
+// part is its own persistent launch whose tail the next one cannot fill, so
+// finer progress costs time (32 parts of an AO-256 pass on a 2048^2 map:
+// +3 %; 8 parts: +0.4 %).
+uint32_t parts_for(uint64_t work) { return (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, work >> 27)); }
+constexpr uint32_t kMappingParts = 8;
 constexpr uint32_t kSamplePermCount = 64 * 64;  // Src/solver_ao.cpp:34
 
 struct MeshPtr {
@@ -58,7 +63,7 @@ public:
 class MappingTask : public Task {
 public:
     explicit MappingTask(fornos_meshmap* mm) : _mm(mm) {}
-    fornos_status submit() override { return fornos_meshmap_submit(_mm, kParts); }
+    fornos_status submit() override { return fornos_meshmap_submit(_mm, kMappingParts); }
     fornos_status poll(float* p, int* d) override { return fornos_meshmap_poll(_mm, p, d); }
     fornos_status finish() override { return FORNOS_OK; }
     const char* name() const override { return "Mesh mapping"; }
@@ -70,12 +75,17 @@ private:
 // The solver tasks (Src/solver_*.cpp, XTask): finish exports the map.
 class SolverTask : public Task {
 public:
-    SolverTask(int kind, fornos_solver* s, std::string path, int dilation, const fornos_solver_params& p)
-        : _kind(kind), _s(s), _path(std::move(path)), _dilation(dilation), _p(p)
+    SolverTask(int kind, fornos_solver* s, std::string path, int dilation, const fornos_solver_params& p,
+               uint32_t texels)
+        : _kind(kind), _s(s), _path(std::move(path)), _dilation(dilation), _p(p), _texels(texels)
     {
     }
     ~SolverTask() override { fornos_solver_destroy(_s); }
-    fornos_status submit() override { return fornos_solver_submit(_s, kParts); }
+    fornos_status submit() override
+    {
+        const uint64_t work = (uint64_t)_texels * (_kind >= FORNOS_SOLVER_AO ? _p.sample_count : 1);
+        return fornos_solver_submit(_s, parts_for(work));
+    }
     fornos_status poll(float* p, int* d) override { return fornos_solver_poll(_s, p, d); }
     fornos_status finish() override
     {
@@ -118,6 +128,7 @@ private:
     std::string _path;
     int _dilation;
     fornos_solver_params _p;
+    uint32_t _texels;
 };
 
 bool has_text(const char* s) { return s && s[0] != '\0'; }
@@ -301,6 +312,8 @@ fornos_status fornos_runner_start(fornos_runner* r, const fornos_parameters* p)
                            {&p->positions, FORNOS_SOLVER_POSITION},  {&p->height, FORNOS_SOLVER_HEIGHT}};
     std::vector<float> table;
     std::vector<uint32_t> rnd;
+    uint32_t texels = 0;
+    if (fornos_status st = fornos_meshmap_texel_count(r->mm, &texels)) return st;
     for (const Spec& sp : specs) {
         const fornos_params_map& m = *sp.m;
         if (!m.enabled) continue;
@@ -324,7 +337,8 @@ fornos_status fornos_runner_start(fornos_runner* r, const fornos_parameters* p)
                                                     hemi ? kSamplePermCount : 0, &s))
             return st;
         const int dilation = sp.kind == FORNOS_SOLVER_POSITION ? 0 : sh.tex_dilation;
-        r->tasks.emplace_back(new SolverTask(sp.kind, s, has_text(m.output_path) ? m.output_path : "", dilation, sprm));
+        r->tasks.emplace_back(
+            new SolverTask(sp.kind, s, has_text(m.output_path) ? m.output_path : "", dilation, sprm, texels));
     }
     // AO and bent normals casting the same rays share one traversal: the AO
     // task (run first) computes both, the BN task finds its work done
