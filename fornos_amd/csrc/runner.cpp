@@ -160,6 +160,7 @@ struct fornos_runner {
         std::future<std::pair<fornos_status, std::string>> result;
     };
     std::vector<Finishing> finishing;
+    const char* last = nullptr;  // name of the task handed to `finishing` last
     std::vector<std::pair<int, std::pair<float, float>>> ranges;  // exported float maps
     // Collects the finishes that have ended (wait: all of them); the first
     // failure is returned with its message.
@@ -186,6 +187,7 @@ struct fornos_runner {
     void clear()
     {
         (void)collect(true);  // exports still reading results end first
+        last = nullptr;
         tasks.clear();  // solvers before the mapping and the scene they use
         fornos_meshmap_destroy(mm);
         mm = nullptr;
@@ -378,6 +380,7 @@ fornos_status fornos_runner_run(fornos_runner* r)
     std::unique_ptr<Task> task = std::move(r->tasks.back());
     r->tasks.pop_back();
     Task* raw = task.get();
+    r->last = raw->name();
     r->finishing.push_back({std::move(task), std::async(std::launch::async, [raw] {
                                 const fornos_status st = raw->finish();
                                 return std::make_pair(st, std::string(st ? fornos_last_error() : ""));
@@ -404,8 +407,8 @@ fornos_status fornos_runner_current_task(fornos_runner* r, const char** name, fl
 {
     if (!r) return set_error(FORNOS_INVALID_ARG, "fornos_runner_current_task: null handle");
     if (r->tasks.empty()) {
-        // a finishing task (its export) is still the current one
-        if (name) *name = r->finishing.empty() ? nullptr : r->finishing.back().task->name();
+        // while exports still run, the last task handed over stays the current one
+        if (name) *name = r->finishing.empty() ? nullptr : r->last;
         if (progress) *progress = 1.0f;
         return FORNOS_OK;
     }
