@@ -176,22 +176,22 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
 
 // One node of the 4-wide culling tree (or one stack pop); STK stack entries.
 template <int KIND, bool COUNT, int STK>
-__device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, float* stk_t, uint32_t& cn,
-                                 uint32_t& nov)
+__device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, uint32_t& cn, uint32_t& nov)
 {
     const uint32_t tid = threadIdx.x;
     const float T = KIND == kThick ? fminf(L.best, A.maxd) : A.maxd;
     if (L.cur == kNoChild) {
         // pop, and visit the popped node in the same step (a popped leaf
-        // ends the walk; a dead thickness entry takes another step)
+        // ends the walk).  Thickness keeps no entry distances on the stack:
+        // a subtree entered beyond its best hit is visited and all of its
+        // children fail the test against that bound, which costs less than
+        // the occupancy a second LDS stack takes.
         if (L.sp == 0) {
             finish<KIND>(A, L, false);
             return;
         }
         --L.sp;
         L.cur = stk_node[L.sp * kBlock + tid];
-        // thickness: a stacked subtree entered beyond the best hit is dead
-        if (KIND == kThick && stk_t[L.sp * kBlock + tid] >= T) L.cur = kNoChild;
         if (L.cur == kNoChild || (L.cur & kLeafFlag)) return;
     }
     const float4* nd = (KIND == kAO ? A.sc.wide_ao : A.sc.wide) + 8 * (size_t)L.cur;
@@ -233,7 +233,6 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
             return;
         }
         stk_node[L.sp * kBlock + tid] = c[k];
-        if (KIND == kThick) stk_t[L.sp * kBlock + tid] = e[k];
         ++L.sp;
     }
 }
@@ -301,7 +300,6 @@ template <int KIND, bool COUNT, bool BATCH, int STK>
 __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __restrict__ jobs, uint32_t njobs)
 {
     __shared__ uint32_t stk_node[STK * kBlock];
-    __shared__ float stk_t[KIND == kThick ? STK * kBlock : 1];
     uint32_t job = 0;
     RayArgs A = BATCH ? jobs[0] : A0;
     uint32_t total = A.ntex * A.S;
@@ -400,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(RayArgs A0, const RayArgs* __re
                     w_lanes += __popcll(wb);
                 }
             }
-            if (walk) wide_step<KIND, COUNT, STK>(A, L, stk_node, stk_t, cn, nov);
+            if (walk) wide_step<KIND, COUNT, STK>(A, L, stk_node, cn, nov);
         }
         // phase 2: parked leaves together; rays on a binary tree take one step
         if (COUNT) {
