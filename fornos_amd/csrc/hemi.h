@@ -296,13 +296,23 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
 // samples, counter and outputs).  A wave works on one job at a time -- the
 // job index is wave-uniform, so its parameters live in scalar registers --
 // and moves on once the job's counter is exhausted and its lanes are idle.
-// Eight waves per SIMD: the compiler is held to 64 VGPRs (AO needs no
-// spill at that budget; BN, thickness and the paired kind spill 16-36 bytes
-// per lane) and the latency of divergent node and triangle fetches is hidden
-// by more waves -- 8-16 % faster than the 6-7 waves the unconstrained
-// register allocation leaves room for.
+// Waves per SIMD the kernel is compiled for: as many as the LDS stack leaves
+// room for (160 KiB per CU; 8 at 16 entries, 6 at 24, 5 at 32).  At 8 the
+// compiler is held to 64 VGPRs (AO needs no spill at that budget; BN,
+// thickness and the paired kind spill 16-36 bytes per lane) and the latency
+// of divergent node and triangle fetches is hidden by more waves -- 8-16 %
+// faster than the 6-7 waves the unconstrained register allocation leaves
+// room for.
+template <int STK>
+constexpr int waves_for_stack()
+{
+    constexpr uint32_t blocks = (160u * 1024u) / (STK * kBlock * 4u);  // per CU, by LDS
+    constexpr uint32_t waves = blocks * (kBlock / 64u) / 4u;            // per SIMD (4 per CU)
+    return waves > 8 ? 8 : (int)waves;
+}
+
 template <int KIND, bool COUNT, bool BATCH, int STK>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_rays(RayArgs A0,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for_stack<STK>()))) void k_rays(RayArgs A0,
                                                                                          const RayArgs* __restrict__ jobs,
                                                                                          uint32_t njobs)
 {
