@@ -143,7 +143,7 @@ __device__ inline int leaf_hits(const RayArgs& A, Lane& L, uint32_t first, uint3
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             float al;
-            if (gms[g] && box_maybe(L.ray, lo[g], hi[g], T, al)) mask |= gms[g];
+            if ((gms[g] != 0) & box_maybe(L.ray, lo[g], hi[g], T, al)) mask |= gms[g];
         }
         if (!mask) return 0;
     }
@@ -208,7 +208,9 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
     for (int k = 0; k < 4; ++k) {
         if (COUNT && chs[k] != kNoChild) ++cn;
         float al = 0;
-        const bool pass = chs[k] != kNoChild && box_maybe(L.ray, lo[k], hi[k], T, al);
+        // `&`, not `&&`: the four tests run without a branch each (an empty
+        // slot's box is in the node and harmless to test)
+        const bool pass = (chs[k] != kNoChild) & box_maybe(L.ray, lo[k], hi[k], T, al);
         e[k] = pass ? al : INFINITY;
         c[k] = pass ? chs[k] : kNoChild;
     }
@@ -221,20 +223,21 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
     FORNOS_CSWAP(0, 1) FORNOS_CSWAP(2, 3) FORNOS_CSWAP(0, 2) FORNOS_CSWAP(1, 3) FORNOS_CSWAP(1, 2)
 #undef FORNOS_CSWAP
     L.cur = c[0];
-#pragma unroll
-    for (int k = 3; k >= 1; --k) {
-        if (c[k] == kNoChild) continue;
-        if (L.sp == STK) {  // redo the ray on the stackless binary culling tree
-            if (COUNT) ++nov;
-            L.slot = kNoChild;
-            L.mode = kCullTree;
-            L.node = 0;
-            L.best = FLT_MAX;
-            return;
-        }
-        stk_node[L.sp * kBlock + tid] = c[k];
-        ++L.sp;
+    // the sort put the passing children first (a failed one sorts last at
+    // INFINITY): c[1..n] go on the stack, the nearest on top
+    const uint32_t n = (uint32_t)(c[1] != kNoChild) + (uint32_t)(c[2] != kNoChild) + (uint32_t)(c[3] != kNoChild);
+    if (L.sp + n > STK) {  // redo the ray on the stackless binary culling tree
+        if (COUNT) ++nov;
+        L.slot = kNoChild;
+        L.mode = kCullTree;
+        L.node = 0;
+        L.best = FLT_MAX;
+        return;
     }
+#pragma unroll
+    for (uint32_t k = 1; k <= 3; ++k)
+        if (k <= n) stk_node[(L.sp + n - k) * kBlock + tid] = c[k];
+    L.sp += n;
 }
 
 // One node of a binary tree in skip-link order: the culling tree (boxes
