@@ -276,6 +276,28 @@ std::vector<uint32_t> reference_parents(const FlatBVH& B)
     return parent;
 }
 
+std::vector<uint32_t> reference_chain_links(const FlatBVH& B)
+{
+    const std::vector<uint32_t> parent = reference_parents(B);
+    std::vector<uint32_t> link(B.n_nodes, 0xFFFFFFFFu);
+    auto word = [&](uint32_t node, int k) {
+        float f;
+        std::memcpy(&f, &B.nodes[8 * (size_t)node + k], 4);
+        return f;
+    };
+    // DFS order: a parent's index is below its children's
+    for (uint32_t x = 0; x < B.n_nodes; ++x) {
+        const uint32_t p = parent[x];
+        if (p == 0xFFFFFFFFu) continue;
+        bool nested = true;
+        for (int k = 0; k < 3; ++k)
+            nested = nested && word(p, k) <= word(x, k) && word(x, k) <= word(x, 4 + k) &&
+                     word(x, 4 + k) <= word(p, 4 + k);
+        link[x] = nested ? link[p] : p;
+    }
+    return link;
+}
+
 namespace {
 
 float fbits(uint32_t w)

@@ -13,7 +13,9 @@
 //          the shader's own operation order
 //   spos   3 x float4 per triangle (reference Vector4 soup positions)
 //   snrm   3 x float4 per triangle (reference Vector4 soup normals)
-//   parent reference parent links; cnodes / wide / wleaf: the culling trees
+//   chain  reachability links of the reference tree (the nearest ancestor
+//          whose test a node's does not imply); cnodes / wide / wleaf: the
+//          culling trees
 // Texel data: pix 2 x float4 per texel (p, d), pixt 3 x float4 (n, t, b),
 // coords float4 (t, b0, b1, b2), tidx uint (soup vertex index).
 #pragma once
@@ -39,7 +41,7 @@ struct DevScene {
     const float4* __restrict__ trest;    // (a, d00), (v0, d01), (v1, d11) per triangle
     const float4* __restrict__ spos;
     const float4* __restrict__ snrm;
-    const uint32_t* __restrict__ parent;  // reference tree parent links
+    const uint32_t* __restrict__ chain;  // reference tree reachability links
     const float4* __restrict__ cnodes;   // culling tree over the reference leaves
     const uint32_t* __restrict__ cref;   // reference leaf of each culling leaf
     const float4* __restrict__ wide;     // the culling tree collapsed to 4 children, 8 float4 per node
@@ -133,9 +135,16 @@ __device__ inline bool node_maybe(const Ray& r, float4 lo, float4 hi, float T)
 // a hit below maxD exists the ray is decided).  So a hit found through the
 // culling tree counts exactly when its reference leaf's chain passes the
 // reference test.
+//
+// Only the leaf and the ancestors its reachability links reach are tested:
+// when a parent's box P contains its child's box C, every rounded step of
+// the test is monotone -- RN(RN(m - o) / d) in the plane m for a fixed d != 0,
+// then min, max -- so P's slab interval contains C's and P passes whenever C
+// does (entry a_P <= a_C < T, exit b_P >= b_C >= max(0, a_C)).  The rays that
+// come here have every |d_i| >= 2^-40, so no quotient is NaN.
 __device__ inline bool chain_ok(const Ray& r, const DevScene& sc, uint32_t leaf, float T)
 {
-    for (uint32_t x = leaf; x != kMiss; x = sc.parent[x])
+    for (uint32_t x = leaf; x != kMiss; x = sc.chain[x])
         if (!node_visit(r, sc.nodes[2 * x], sc.nodes[2 * x + 1], T)) return false;
     return true;
 }

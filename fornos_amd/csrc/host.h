@@ -86,6 +86,13 @@ void export_reference_nodes(const FlatBVH& B, uint32_t* nodes9);  // BVHGPUData,
 // Parent of every reference node (root: 0xFFFFFFFF), from the skip links.
 std::vector<uint32_t> reference_parents(const FlatBVH& B);
 
+// Reachability links: for every reference node x, the nearest ancestor whose
+// slab test is not implied by x's (0xFFFFFFFF: none).  A parent whose box
+// contains its child's box (exact float compares of the stored corners)
+// passes the reference's test whenever the child does, so the root-to-leaf
+// test of a leaf needs only the leaf and the ancestors these links reach.
+std::vector<uint32_t> reference_chain_links(const FlatBVH& B);
+
 // Culling tree over the reference leaves: a binned-SAH binary tree whose
 // leaves are the reference leaves (same triangles), each box the union of
 // the reference leaf box and its triangles, padded by `pad`.  Same 8-word node

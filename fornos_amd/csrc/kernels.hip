@@ -48,7 +48,7 @@ struct fornos_scene {
     hipStream_t stream = nullptr;
     DevScene d{};
     float4 *nodes = nullptr, *tri = nullptr, *spos = nullptr, *snrm = nullptr, *cnodes = nullptr;
-    uint32_t *parent = nullptr, *cref = nullptr;
+    uint32_t *chain = nullptr, *cref = nullptr;
     float4* wide = nullptr;
     uint4* wleaf = nullptr;
     float4* wide_ao = nullptr;  // AO's tree with split leaves (null: AO walks `wide`)
@@ -72,7 +72,7 @@ struct fornos_scene {
         if (xevent) (void)hipEventDestroy(xevent);
         if (xstream) (void)hipStreamDestroy(xstream);
         dev_free(nodes); dev_free(tri); dev_free(spos); dev_free(snrm);
-        dev_free(cnodes); dev_free(parent); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao); dev_free(wsub);
+        dev_free(cnodes); dev_free(chain); dev_free(cref); dev_free(wide); dev_free(wleaf); dev_free(wide_ao); dev_free(wleaf_ao); dev_free(wsub);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -434,19 +434,19 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
         // inside them still spare triangle tests
         if (cull_ao.n_wide) add_leaf_groups(cull, sp4, amax * (1.0f / 65536.0f));
     }
-    const std::vector<uint32_t> parent = reference_parents(B);
+    const std::vector<uint32_t> chain = reference_chain_links(B);
     const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
 
     HIP_TRY(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
     if (fornos_status st = dev_alloc(&sc->cnodes, (size_t)2 * cull.n_nodes)) return st;
     if (fornos_status st = dev_alloc(&sc->cref, cull.ref_leaf.size())) return st;
-    if (fornos_status st = dev_alloc(&sc->parent, parent.size())) return st;
+    if (fornos_status st = dev_alloc(&sc->chain, chain.size())) return st;
     HIP_TRY(hipMemcpy(sc->cnodes, cull.nodes.data(), sizeof(uint32_t) * 8 * (size_t)cull.n_nodes, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sc->cref, cull.ref_leaf.data(), sizeof(uint32_t) * cull.ref_leaf.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sc->parent, parent.data(), sizeof(uint32_t) * parent.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->chain, chain.data(), sizeof(uint32_t) * chain.size(), hipMemcpyHostToDevice));
     sc->d.cnodes = sc->cnodes;
     sc->d.cref = sc->cref;
-    sc->d.parent = sc->parent;
+    sc->d.chain = sc->chain;
     sc->d.n_cnodes = cull.n_nodes;
     if (fornos_status st = dev_alloc(&sc->wide, (size_t)8 * cull.n_wide)) return st;
     if (fornos_status st = dev_alloc(&sc->wleaf, cull.wide_leaves.size() / 4)) return st;
@@ -491,7 +491,7 @@ fornos_status fornos_scene_create(const fornos_mesh_view* hi, const fornos_bvh_p
     sc->info.build_seconds = build_s;
     sc->info.device_bytes = sizeof(float4) * (2 * (size_t)B.n_nodes + tri.size() + spos.size() + snrm.size() +
                                               2 * (size_t)cull.n_nodes) +
-                            sizeof(uint32_t) * (parent.size() + cull.ref_leaf.size() + cull.wide.size() +
+                            sizeof(uint32_t) * (chain.size() + cull.ref_leaf.size() + cull.wide.size() +
                                                 cull.wide_leaves.size() + cull_ao.wide.size() +
                                                 cull_ao.wide_leaves.size() + cull.sub.size());
     sc->info.ao_groups = cull_ao.n_groups;
