@@ -183,6 +183,12 @@ def main():
                 "reduce_ms_per_step": float(np.mean(reduce_ms)), "pass_kernel_ms": float(np.mean(pass_ms)),
                 "node_tests_per_ray": work["node_tests"] / max(1, work["rays"]),
                 "tri_tests_per_ray": work["tri_tests"] / max(1, work["rays"])}
+    if traffic:
+        # the algorithmic bytes count every node and triangle fetch, most of
+        # which L2 and the MALL serve, so `frac` can pass 1; DRAM bytes from
+        # the rocprofv3 counters over the same launch time cannot
+        roofline["dram_gbs"] = traffic / (launch_ms * 1e-3) / 1e9
+        roofline["dram_frac"] = roofline["dram_gbs"] / HBM_PEAK_GBS
 
     bake = None
     if not args.no_bake:
