@@ -365,7 +365,7 @@ class Solver(_Handle):
             w[k] = int(v)
         return w
 
-    def set_launch(self, blocks_per_cu=8, refill=32, stack=16):
+    def set_launch(self, blocks_per_cu=8, refill=48, stack=16):
         check(lib().fornos_solver_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
 
     def get_launch(self):

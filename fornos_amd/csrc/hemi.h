@@ -45,7 +45,7 @@ enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ra
 // walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes;
 // then rays traced on the reference tree (slow directions and thickness redos)
 constexpr uint32_t kStats = 11;
-constexpr uint32_t kRefill = 32;  // default: refill a wave once this many lanes are idle
+constexpr uint32_t kRefill = 48;  // default: refill a wave once this many lanes are idle
 constexpr uint32_t kQuorum = 8;   // leaf step once searching lanes <= parked lanes / kQuorum (0: wait for all)
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
 

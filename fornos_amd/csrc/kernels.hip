@@ -1304,7 +1304,7 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
         return FORNOS_OK;
     };
     // coordinate descent: stack depth, then blocks per CU, then refill threshold
-    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {8, 16, 32}, {16, 24, 32}};
+    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {32, 40, 48, 56}, {16, 24, 32}};
     float best = FLT_MAX;
     for (int axis : {2, 0, 1}) {
         uint32_t pick = cur[axis];

@@ -261,7 +261,7 @@ fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *coun
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
  * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
  * stack entries per lane (16, 24 or 32; a ray that would overflow it is
- * redone on a stackless tree, so results do not depend on it); default 8, 32,
+ * redone on a stackless tree, so results do not depend on it); default 8, 48,
  * 16.  No reference counterpart (the shaders' dispatch shape is fixed). */
 fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu, uint32_t refill,
                                        uint32_t stack);
@@ -286,7 +286,7 @@ fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_pe
                                       uint32_t *stack);
 /* Autotuner: times launch shapes on the first `fraction` of each job's
  * texels (at least 2^22 rays of each job) by coordinate descent (stack depth {16,24,32}, then blocks per CU
- * {4,6,8,12}, then refill {8,16,32}) and keeps the fastest. */
+ * {4,6,8,12}, then refill {32,40,48,56}) and keeps the fastest. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
                                     uint32_t *refill, uint32_t *stack, float *best_ms);
 /* Runs every job once (graph launch); *ms = wall time of the graph on the

@@ -199,7 +199,7 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     cls = {"ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
            "thickness": fa.ThicknessSolver}[kind]
     s = cls(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
-    assert s.get_launch() == (8, 32, 16)
+    assert s.get_launch() == (8, 48, 16)
     s.run_all()
     ref = s.getResults().view(np.uint32).copy()
     live = int(np.sum(field.tidx != MISS))
