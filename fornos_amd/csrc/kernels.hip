@@ -847,13 +847,29 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         s->P = n_perm;
         s->work_total = (uint64_t)mm->work_count * S;
         {
+            // FNV-1a over the table's 32-bit words in blocks of 2^16 words
+            // (hashed on several threads), then over the block hashes in order
+            const size_t words = 3 * (size_t)n_perm * S, block = 1u << 16;
+            std::vector<uint64_t> bh((words + block - 1) / block);
+            parallel_chunks(bh.size(), [&](size_t b0, size_t b1) {
+                for (size_t b = b0; b < b1; ++b) {
+                    uint64_t h = 1469598103934665603ull;
+                    for (size_t i = b * block; i < std::min(words, (b + 1) * block); ++i) {
+                        uint32_t w;
+                        std::memcpy(&w, &table[i], 4);
+                        h = (h ^ w) * 1099511628211ull;
+                    }
+                    bh[b] = h;
+                }
+            }, 1);
             uint64_t h = 1469598103934665603ull;
-            const uint8_t* b = (const uint8_t*)table;
-            for (size_t i = 0; i < sizeof(float) * 3 * (size_t)n_perm * S; ++i) h = (h ^ b[i]) * 1099511628211ull;
+            for (uint64_t x : bh) h = (h ^ x) * 1099511628211ull;
             s->table_hash = h;
         }
         std::vector<float4> t4((size_t)n_perm * S);
-        for (size_t i = 0; i < t4.size(); ++i) t4[i] = make_float4(table[3 * i], table[3 * i + 1], table[3 * i + 2], 0.0f);
+        parallel_chunks(t4.size(), [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) t4[i] = make_float4(table[3 * i], table[3 * i + 1], table[3 * i + 2], 0.0f);
+        }, 1u << 16);
         if (fornos_status st = dev_alloc(&s->samples, t4.size())) return st;
         HIP_TRY(hipMemcpy(s->samples, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));
         // ray order: each permutation's samples sorted by the Morton code of
