@@ -105,6 +105,7 @@ _SIGS = [
     ("fornos_texels_export_mapuv", None, [H, P, P, P, P, P]),
     ("fornos_texels_destroy", "void", [H]),
     ("fornos_bvh_reference_export", None, [H, C.c_uint32, C.c_uint32, u32p, P, P]),
+    ("fornos_bvh_reference_links", None, [H, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
     ("fornos_sample_table", None, [C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
     # device
     ("fornos_scene_create", None, [C.POINTER(MeshView), C.POINTER(BVHParams), C.POINTER(H)]),

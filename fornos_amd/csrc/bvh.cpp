@@ -810,3 +810,23 @@ extern "C" fornos_status fornos_bvh_reference_export(const fornos_mesh* m, uint3
     *n_nodes = B.n_nodes;
     return FORNOS_OK;
 }
+
+extern "C" fornos_status fornos_bvh_reference_links(const fornos_mesh* m, uint32_t max_tris, uint32_t max_depth,
+                                                    uint32_t n_nodes, uint32_t* parent, uint32_t* links)
+{
+    if (!m || max_tris == 0 || (!parent && !links))
+        return fornos::set_error(FORNOS_INVALID_ARG, "fornos_bvh_reference_links: bad argument");
+    fornos::FlatBVH B;
+    fornos::build_reference_bvh(*m, max_tris, max_depth, B);
+    if (n_nodes != B.n_nodes)
+        return fornos::set_error(FORNOS_INVALID_ARG, "fornos_bvh_reference_links: node count differs from the tree's");
+    if (parent) {
+        const std::vector<uint32_t> p = fornos::reference_parents(B);
+        std::memcpy(parent, p.data(), sizeof(uint32_t) * p.size());
+    }
+    if (links) {
+        const std::vector<uint32_t> l = fornos::reference_chain_links(B);
+        std::memcpy(links, l.data(), sizeof(uint32_t) * l.size());
+    }
+    return FORNOS_OK;
+}

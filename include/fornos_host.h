@@ -96,6 +96,17 @@ fornos_status fornos_bvh_reference_export(const fornos_mesh *m, uint32_t max_tri
                                           uint32_t max_depth, uint32_t *n_nodes,
                                           uint32_t *nodes9, uint32_t *soup_triangle_ids);
 
+/* The same tree's parent links and the reachability links the hemisphere
+ * kernels walk (each node's nearest ancestor whose slab test its own does
+ * not imply, as the parent's stored box does not contain the node's);
+ * 0xFFFFFFFF above the root.  n_nodes must be the tree's node count
+ * (fornos_bvh_reference_export); either output may be NULL.  No reference
+ * counterpart: the shaders test every node from the root down
+ * (Shaders/ao_step1.comp:111-138). */
+fornos_status fornos_bvh_reference_links(const fornos_mesh *m, uint32_t max_tris_per_leaf,
+                                         uint32_t max_depth, uint32_t n_nodes, uint32_t *parent,
+                                         uint32_t *links);
+
 /* Hemisphere sample table: n_perm * S cosine-weighted directions (3 floats)
  * from hammersley(i, S, randX, randY) (Src/math.h:339-382).  (randX, randY)
  * pairs are drawn from a seeded std::mt19937 (two raw outputs per
