@@ -85,10 +85,13 @@ def denoise_leg(prep):
     job = dn.MeshJob(spec, cfg["size"], (prep["low"], prep["dirs"], prep["high"], prep["scene"]))
     net = dn.Denoiser.load(dn.WEIGHTS)
     net.forward(torch.zeros((1, cfg["size"], cfg["size"], dn.IN), dtype=torch.float32, device="cuda"))
-    ra, rb, tp, mm = dn.plain_bake(job)
+    # both bakes read their maps back into page-locked buffers allocated here
+    plain_host = dn.readback_buffers(cfg["size"] * cfg["size"])
+    dn_host = dn.readback_buffers(cfg["size"] * cfg["size"])
+    ra, rb, tp, mm = dn.plain_bake(job, host=plain_host)
     _, tidx = mm.read()
     hit = tidx != 0xFFFFFFFF
-    da, db, td, info = dn.denoised_bake(job, net)
+    da, db, td, info = dn.denoised_bake(job, net, host=dn_host)
     ae, be = dn.errors(da, db, ra, rb, hit)
     print("# denoise: plain %.2f s, denoised %.3f s, AO MAE %.4f, BN %.2f deg" % (tp, td, ae, be),
           file=sys.stderr, flush=True)

@@ -336,10 +336,16 @@ class Solver(_Handle):
     def reset(self):
         check(lib().fornos_solver_reset(self._h))
 
-    def getResults(self):
+    def getResults(self, out=None):
+        """The solver's results (one float or three per texel); `out`, when
+        given, is the float32 host array to fill (page-locked memory makes
+        the read a plain DMA)."""
         n = self.meshMapping.count
         shape = (n,) if self.comps == 1 else (n, self.comps)
-        out = np.empty(shape, np.float32)
+        if out is None:
+            out = np.empty(shape, np.float32)
+        elif out.shape != shape or out.dtype != np.float32 or not out.flags["C_CONTIGUOUS"]:
+            raise ValueError("getResults: out must be a C-contiguous float32 array of shape %s" % (shape,))
         check(lib().fornos_solver_read(self._h, ptr(out), out.nbytes))
         return out
 

@@ -219,22 +219,36 @@ def training_pair(job, x0, y0, w, seed):
     return inp, tgt[..., :OUT].contiguous()
 
 
-def plain_bake(job, S=HIGH_S, seed=7):
-    """Mapping + AO + bent normals at S samples; host results and wall time."""
+def readback_buffers(n):
+    """Page-locked host arrays (AO [n], bent normals [n, 3]) for the results of
+    a timed bake of at most n texels, allocated before the timing starts, so
+    that both bakes of a comparison read back with the same plain DMA."""
+    torch = _torch()
+    return (torch.empty(n, dtype=torch.float32, pin_memory=True),
+            torch.empty((n, 3), dtype=torch.float32, pin_memory=True))
+
+
+def plain_bake(job, S=HIGH_S, seed=7, host=None):
+    """Mapping + AO + bent normals at S samples; host results and wall time.
+    host: readback_buffers to read into (default: new arrays)."""
     t0 = time.perf_counter()
     mm = job.mapping()
     ao, bn = job.solvers(mm, S, seed)
     ao.run_all()
     bn.run_all()
-    a, b = ao.getResults(), bn.getResults()
+    if host is None:
+        a, b = ao.getResults(), bn.getResults()
+    else:
+        n = mm.count
+        a, b = ao.getResults(out=host[0][:n].numpy()), bn.getResults(out=host[1][:n].numpy())
     t = time.perf_counter() - t0
     return a, b, t, mm
 
 
-def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True):
+def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True, host=None):
     """Mapping + 16-sample AO / bent normals + G-buffers -> network -> per-tile
     check -> plain 1024-sample bake of the failing tiles.  Host results, wall
-    time and a breakdown."""
+    time and a breakdown.  host: readback_buffers to read into."""
     torch = _torch()
     W = job.size
     t0 = time.perf_counter()
@@ -255,7 +269,13 @@ def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True)
     nt = (W + tile - 1) // tile
     fail = torch.empty(nt * nt, dtype=torch.int32, device="cuda")
     check(lib().fornos_denoise_check_tiles(_p(inp), _p(out), W, W, tile, LOW_S, z, slack, _p(fail)))
-    a, b = d_ao.cpu().numpy(), d_bn.cpu().numpy()
+    if host is None:
+        a, b = d_ao.cpu().numpy(), d_bn.cpu().numpy()
+    else:
+        a, b = host[0][:n], host[1][:n]
+        a.copy_(d_ao)
+        b.copy_(d_bn)
+        a, b = a.numpy(), b.numpy()
     bad = fail.cpu().numpy().reshape(nt, nt).astype(bool)
     t3 = time.perf_counter()
     n_fb = 0
@@ -442,13 +462,14 @@ def main(argv=None):
     net = Denoiser.load(args.weights)
     net.forward(torch.zeros((1, args.size, args.size, IN), dtype=torch.float32, device="cuda"))  # allocations
     rows = []
+    plain_host, dn_host = readback_buffers(args.size * args.size), readback_buffers(args.size * args.size)
     for spec in _specs(_seed_range(args.seeds)):
         job = MeshJob(spec, args.size)
-        ra, rb, tp, mm = plain_bake(job)
+        ra, rb, tp, mm = plain_bake(job, host=plain_host)
         _, tidx = mm.read()
         hit = tidx != 0xFFFFFFFF
         a16, b16, _, _ = plain_bake(job, LOW_S, seed=11)
-        da, db, td, info = denoised_bake(job, net)
+        da, db, td, info = denoised_bake(job, net, host=dn_host)
         e16 = errors(a16, b16, ra, rb, hit)
         edn = errors(da, db, ra, rb, hit)
         row = {"seed": spec["seed"], "family": spec["family"], "high_tris": job.high_tris,
