@@ -78,7 +78,7 @@ void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<ui
 }
 
 // PNG image data as one zlib stream, built in parallel: the rows are split
-// into segments of about 1 MiB, each worker filters its rows (PNG filter 1,
+// into segments of about 256 KiB, each worker filters its rows (PNG filter 1,
 // "Sub": byte minus the byte 3 to the left, mod 256) and deflates them as a
 // raw deflate run of its own ending on a byte-aligned sync-flush block (the
 // last segment ends the stream).  A run never refers back into another
@@ -87,7 +87,7 @@ void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<ui
 bool png_idat(const uint8_t* rgb, uint32_t width, uint32_t height, int level, std::vector<uint8_t>& z)
 {
     const size_t stride = 3 * (size_t)width, line = stride + 1;
-    const uint32_t rows_per = (uint32_t)std::max<size_t>(1, ((size_t)1 << 20) / line);
+    const uint32_t rows_per = (uint32_t)std::max<size_t>(1, ((size_t)1 << 18) / line);
     const uint32_t nseg = (height + rows_per - 1) / rows_per;
     std::vector<std::vector<uint8_t>> part(nseg);
     std::vector<uLong> adl(nseg);

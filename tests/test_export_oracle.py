@@ -112,7 +112,7 @@ def test_exr_round_trip(tmp_path):
 
 @pytest.mark.parametrize("w,h", [(1200, 1000), (1, 1), (700, 3)])
 def test_png_segments_form_one_zlib_stream(tmp_path, w, h):
-    # the writer deflates ~1 MiB row segments on several threads; the IDAT
+    # the writer deflates ~256 KiB row segments on several threads; the IDAT
     # must still be one zlib stream with the right Adler-32 (zlib checks it)
     # and Sub-filtered rows that decode to the input
     import struct
