@@ -46,11 +46,17 @@ enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ra
 // then rays traced on the reference tree (slow directions and thickness redos)
 constexpr uint32_t kStats = 11;
 constexpr uint32_t kRefill = 48;  // default: refill a wave once this many lanes are idle
-constexpr uint32_t kQuorum = 8;   // leaf step once searching lanes <= parked lanes / kQuorum (0: wait for all)
-// Thickness lanes that hold a leaf do not walk on (its pruning needs the
-// hits first), so they idle while others search: its leaf step starts once
-// a third of the lanes still search (knot4096 thickness 7 % faster).
-constexpr uint32_t kQuorumThick = 2;
+// Leaf step once the lanes still searching are at most the lanes holding a
+// leaf / quorum (0: wait for all).  Thickness lanes that hold a leaf do not
+// walk on (its pruning needs the hits first), so they idle while others
+// search: its step starts once a third of the lanes still search (knot4096
+// thickness 7 % faster than at 8).  Bent normals and the paired pass gain
+// 1-2 % at 4; AO is within noise from 4 to 16.
+template <int KIND>
+constexpr uint32_t quorum()
+{
+    return KIND == kThick ? 2u : (KIND == kAO ? 8u : 4u);
+}
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
 
 struct RayArgs {
@@ -405,12 +411,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
             }
             const uint64_t need = __ballot(wide && L.slot == kNoChild);
             if (need == 0) break;
-            if (kQuorum) {
+            if (quorum<KIND>()) {
                 // leaf step once few lanes are still searching (they go on
                 // walking after it); never without a parked leaf
                 const uint64_t parked = __ballot(wide && L.slot != kNoChild);
-                const uint32_t quorum = KIND == kThick ? kQuorumThick : kQuorum;
-                if (parked != 0 && __popcll(need) * quorum <= (uint32_t)__popcll(parked)) break;
+                if (parked != 0 && __popcll(need) * quorum<KIND>() <= (uint32_t)__popcll(parked)) break;
             }
             // thickness does not walk past its leaf: its pruning needs the hits first
             const bool walk = wide && (KIND != kThick || L.slot == kNoChild) &&
