@@ -50,15 +50,19 @@ __device__ inline bool map_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float
 
 // One pass of raycastBVH / raycastBackBVH in the reference's stackless DFS
 // order (Shaders/meshmapping_nobackfaces.comp:102-121): visit a node iff its
-// slab entry is < the running best, else jump past its subtree.
+// slab entry is < the running best, else jump past its subtree.  The node
+// test is node_visit: the reference's RayAABB(...) < cur decided on the
+// reciprocal-multiply slab distances wherever they leave no doubt, with the
+// reference's divisions for the rest.
 template <int MODE>
-__device__ inline void map_pass(V3 o, V3 d, const DevScene& sc, float& cur, uint32_t& idx, V3& bc)
+__device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, uint32_t& idx, V3& bc)
 {
+    const V3 o = r.o, d = r.d;
     uint32_t i = 0;
     while (i < sc.n_nodes) {
         const float4 lo = sc.nodes[2 * i], hi = sc.nodes[2 * i + 1];
         const uint32_t w = __float_as_uint(hi.w);
-        if (ray_aabb(o, d, lo, hi) < cur) {
+        if (node_visit(r, lo, hi, cur)) {
             if (w & kLeafFlag) {
                 const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
                 for (uint32_t k = first; k < end; ++k) {
@@ -86,14 +90,22 @@ __global__ __launch_bounds__(kBlock) void k_meshmap(const float4* __restrict__ p
     uint32_t idx = kMiss;
     V3 bc = mk(0, 0, 0);
     float t = FLT_MAX;
+    Ray fw, bw;
+    fw.o = bw.o = p;
+    fw.d = d;
+    bw.d = -d;
+    fw.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    bw.inv = -fw.inv;  // 1/(-x) = -(1/x) exactly
+    fw.oi = bw.oi = mk(0, 0, 0);  // (used by the culling test only)
+    fw.slow = bw.slow = !(fabsf(d.x) >= kSlowDir && fabsf(d.y) >= kSlowDir && fabsf(d.z) >= kSlowDir);
     if (CULL) {
         // meshmapping_nobackfaces.comp:182-199: one running t shared by both passes
-        map_pass<1>(p, d, sc, t, idx, bc);
-        map_pass<2>(p, -d, sc, t, idx, bc);
+        map_pass<1>(fw, sc, t, idx, bc);
+        map_pass<2>(bw, sc, t, idx, bc);
     } else {
         // meshmapping.comp:139-162: the backward pass starts from the forward t
-        map_pass<0>(p, d, sc, t, idx, bc);
-        map_pass<0>(p, -d, sc, t, idx, bc);
+        map_pass<0>(fw, sc, t, idx, bc);
+        map_pass<0>(bw, sc, t, idx, bc);
     }
     coords[g] = make_float4(t, bc.x, bc.y, bc.z);
     tidx[g] = idx;
