@@ -78,8 +78,10 @@ __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, ui
     }
 }
 
+// Eight waves per SIMD (57 VGPRs, no spill; the compiler's own choice of 66
+// leaves room for 7): more texels in flight hide the dependent node loads.
 template <bool CULL>
-__global__ __launch_bounds__(kBlock) void k_meshmap(const float4* __restrict__ pix, uint32_t begin,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_meshmap(const float4* __restrict__ pix, uint32_t begin,
                                                      uint32_t end, DevScene sc,
                                                      float4* __restrict__ coords,
                                                      uint32_t* __restrict__ tidx)
