@@ -113,7 +113,7 @@ def main():
                          "that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+        print("bench.py: the path runs on one GPU only; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
