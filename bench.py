@@ -42,12 +42,15 @@ def load_traffic(workload):
 def cpu_baseline(prep, mm, solver, budget_s=15.0):
     """The oracle (C restatement of the shaders, OpenMP over texels) on a
     bounded sample of the same AO-256 workload: contiguous 256-texel chunks
-    spread over the map until ~budget_s of wall time."""
+    spread over the map until ~budget_s of wall time.  Its per-texel occluder
+    counts and AO values are compared with the GPU pass's on the same texels."""
     import numpy as np
     import oracle as orc
     nodes, _ = prep["scene"].reference_nodes()
     spos, snrm = prep["scene"].soup()
     coords, tidx = mm.read()
+    gpu_counts = solver.counts()
+    gpu_ao = solver.getResults().view(np.uint32)
     p = prep["cfg"]["maps"]["ao"]
     S = p["sampleCount"]
     n = len(tidx)
@@ -55,19 +58,25 @@ def cpu_baseline(prep, mm, solver, budget_s=15.0):
     starts = np.linspace(0, n - 256, 1024).astype(int)
     order = np.concatenate([starts[k::32] for k in range(32)])
     rays, texels, t = 0, 0, 0.0
+    count_mismatch, value_mismatch = 0, 0
     for s in order:
         t0 = time.perf_counter()
-        orc.hemisphere(orc.KIND_AO, S, p["minDistance"], p["maxDistance"], solver.table, coords, tidx,
-                       spos, snrm, nodes, int(s), int(s) + 256, nthreads=threads)
+        out, cnt = orc.hemisphere(orc.KIND_AO, S, p["minDistance"], p["maxDistance"], solver.table, coords, tidx,
+                                  spos, snrm, nodes, int(s), int(s) + 256, nthreads=threads)
         t += time.perf_counter() - t0
+        count_mismatch += int(np.sum(cnt != gpu_counts[s:s + 256]))
+        value_mismatch += int(np.sum(out.view(np.uint32) != gpu_ao[s:s + 256]))
         live = int(np.sum(tidx[s:s + 256] != 0xFFFFFFFF))
         rays += live * S
         texels += 256
         if t > budget_s:
             break
-    return {"value": rays / t, "unit": "rays/s", "cores": threads, "kind": "port",
+    base = {"value": rays / t, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": "%d texels (%d-texel chunks spread over the map) x %d AO samples = %d rays, %.1f s"
                       % (texels, 256, S, rays, t)}
+    parity = {"texels": texels, "rays": rays, "count_mismatches": count_mismatch,
+              "value_mismatches": value_mismatch, "against": "oracle, same texels and sample table"}
+    return base, parity
 
 
 def denoise_leg(prep):
@@ -209,7 +218,7 @@ def main():
                 "paired": rep.get("paired")}
 
     denoise = None if args.no_denoise else denoise_leg(prep)
-    cpu = None if args.no_cpu else cpu_baseline(prep, mm, ao)
+    cpu, parity = (None, None) if args.no_cpu else cpu_baseline(prep, mm, ao)
     line = {
         "metric": "AO rays/s", "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -219,10 +228,14 @@ def main():
                                   p["maxDistance"]),
                    "texels": n, "rays_per_step": rays_per_step,
                    "bvh": "reference binary tree (exact reachability) + 4-wide SAH culling tree"},
-        "roofline": roofline, "cpu_baseline": cpu, "mapping_ms": map_ms, "bake": bake, "denoise": denoise,
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "mapping_ms": map_ms, "bake": bake, "denoise": denoise,
         "texels_per_s": live / (ms_per_step * 1e-3),
     }
     print(json.dumps(line), flush=True)
+    if parity and (parity["count_mismatches"] or parity["value_mismatches"]):
+        print("bench.py: the AO pass differs from the oracle on %d texels" % parity["count_mismatches"],
+              file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

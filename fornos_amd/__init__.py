@@ -7,10 +7,10 @@ the seeded bake workloads.
 from . import _abi
 from ._abi import FornosError, SYMBOLS, lib
 from .api import (AmbientOcclusionSolver, Batch, BentNormalsSolver, CompressedMapUV, HeightSolver, Mesh,
-                  MeshMapping, NormalsSolver, PositionSolver, Scene, TexelArrays, ThicknessSolver,
+                  MeshMapping, NormalsSolver, PositionSolver, Scene, Solver, TexelArrays, ThicknessSolver,
                   device_count, sample_table)
 from .configs import CONFIGS, prepare
 
 __all__ = ["AmbientOcclusionSolver", "Batch", "BentNormalsSolver", "CompressedMapUV", "HeightSolver", "Mesh",
-           "MeshMapping", "NormalsSolver", "PositionSolver", "Scene", "TexelArrays", "ThicknessSolver",
+           "MeshMapping", "NormalsSolver", "PositionSolver", "Scene", "Solver", "TexelArrays", "ThicknessSolver",
            "device_count", "sample_table", "CONFIGS", "prepare", "FornosError", "SYMBOLS", "lib"]

@@ -363,13 +363,25 @@ class Solver(_Handle):
     def count_work(self):
         a, b, c, d = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().fornos_solver_count_work(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
-        out = np.zeros(11, np.uint64)
-        check(lib().fornos_solver_work_counters(self._h, ptr(out), 11))
+        out = np.zeros(12, np.uint64)
+        check(lib().fornos_solver_work_counters(self._h, ptr(out), 12))
         w = {"node_tests": a.value, "tri_tests": b.value, "rays": c.value, "stack_overflows": d.value}
         for k, v in zip(["walk_steps", "walk_lanes", "leaf_steps", "leaf_lanes", "refills", "refill_lanes",
-                         "reference_tree_rays"], out[4:]):
+                         "redo_rays", "slow_rays"], out[4:]):
             w[k] = int(v)
+        w["reference_tree_rays"] = w["redo_rays"] + w["slow_rays"]
         return w
+
+    FB_OVERFLOW, FB_REDO, FB_SLOW = 1, 2, 4
+
+    def fallback_flags(self):
+        """Per-texel flags of the last count_work run: FB_OVERFLOW (a ray was
+        redone on the stackless tree), FB_REDO (a thickness ray was redone on
+        the reference tree), FB_SLOW (a ray was traced on the reference tree
+        for a direction component below 2^-40)."""
+        out = np.empty(self.meshMapping.count, np.uint32)
+        check(lib().fornos_solver_fallback_flags(self._h, ptr(out)))
+        return out
 
     def set_launch(self, blocks_per_cu=8, refill=48, stack=16):
         check(lib().fornos_solver_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))

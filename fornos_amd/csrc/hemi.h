@@ -43,8 +43,11 @@ enum { kAO = 0, kBN = 1, kThick = 2, kAOBN = 3 };  // kAOBN: AO and bent normals
 enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ray
 // counting run: box tests, triangle tests, rays, stack overflows, then per wave:
 // walk steps, walking lanes, leaf steps, leaf lanes, refills, refilled lanes;
-// then rays traced on the reference tree (slow directions and thickness redos)
-constexpr uint32_t kStats = 11;
+// then thickness rays redone on the reference tree, and rays traced there from
+// the start (a direction component below 2^-40)
+constexpr uint32_t kStats = 12;
+// counting run, per texel: which fallback paths its rays took
+enum : uint32_t { kFbOverflow = 1u, kFbRedo = 2u, kFbSlow = 4u };
 constexpr uint32_t kRefill = 48;  // default: refill a wave once this many lanes are idle
 // Leaf step once the lanes still searching are at most the lanes holding a
 // leaf / quorum (0: wait for all).  Thickness lanes that hold a leaf do not
@@ -69,6 +72,7 @@ struct RayArgs {
     float* __restrict__ tval;           // thickness: closest t per ray of the launch
     unsigned int* __restrict__ counter;  // next ray of the launch to hand out
     unsigned long long* __restrict__ stats;  // kStats counters (COUNT only)
+    uint32_t* __restrict__ fb;          // kFb* flags per texel, global texel index (COUNT only)
     uint32_t S, P, t0, ntex, mwords;    // texels [t0, t0 + ntex), S rays each
     float mind, maxd;
     uint32_t refill;                    // refill a wave once this many lanes are idle
@@ -237,7 +241,10 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
     // INFINITY): c[1..n] go on the stack, the nearest on top
     const uint32_t n = (uint32_t)(c[1] != kNoChild) + (uint32_t)(c[2] != kNoChild) + (uint32_t)(c[3] != kNoChild);
     if (L.sp + n > STK) {  // redo the ray on the stackless binary culling tree
-        if (COUNT) ++nov;
+        if (COUNT) {
+            ++nov;
+            atomicOr(&A.fb[A.t0 + L.q], kFbOverflow);
+        }
         L.slot = kNoChild;
         L.mode = kCullTree;
         L.node = 0;
@@ -290,7 +297,10 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
                     return;
                 }
                 if (r == 2) {
-                    if (COUNT) ++nref;
+                    if (COUNT) {
+                        ++nref;
+                        atomicOr(&A.fb[A.t0 + L.q], kFbRedo);
+                    }
                     L.mode = kRefTree;
                     L.node = 0;
                     L.best = FLT_MAX;
@@ -344,7 +354,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     L.active = false;
     L.bn = false;
     bool exhausted = false;
-    uint32_t cn = 0, ct = 0, nr = 0, nov = 0, nref = 0;
+    uint32_t cn = 0, ct = 0, nr = 0, nov = 0, nref = 0, nslow = 0;
     unsigned long long w_steps = 0, w_lanes = 0, l_steps = 0, l_lanes = 0, r_steps = 0, r_lanes = 0;
     for (;;) {
         // refill idle lanes, one atomic per wave
@@ -375,7 +385,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                         L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
                                        fabsf(L.ray.d.z) >= kSlowDir);
                         L.mode = L.ray.slow ? kRefTree : kWideTree;
-                        if (COUNT && L.ray.slow) ++nref;
+                        if (COUNT && L.ray.slow) {
+                            ++nslow;
+                            atomicOr(&A.fb[g], kFbSlow);
+                        }
                         L.q = q;
                         L.s = __float_as_uint(rs.w);  // the sample's index in the table
                         L.cur = 0;
@@ -448,7 +461,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 if (r == 1) {
                     finish<KIND>(A, L, true);
                 } else if (r == 2) {
-                    if (COUNT) ++nref;
+                    if (COUNT) {
+                        ++nref;
+                        atomicOr(&A.fb[A.t0 + L.q], kFbRedo);
+                    }
                     L.mode = kRefTree;
                     L.node = 0;
                     L.best = FLT_MAX;
@@ -457,13 +473,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
         }
     }
     if (COUNT) {
-        unsigned long long a = cn, b = ct, c = nr, o = nov, f = nref;
+        unsigned long long a = cn, b = ct, c = nr, o = nov, f = nref, sl = nslow;
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_down(a, off);
             b += __shfl_down(b, off);
             c += __shfl_down(c, off);
             o += __shfl_down(o, off);
             f += __shfl_down(f, off);
+            sl += __shfl_down(sl, off);
         }
         if (lane == 0) {
             atomicAdd(&A.stats[4], w_steps);
@@ -474,6 +491,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
             atomicAdd(&A.stats[9], r_lanes);
             atomicAdd(&A.stats[3], o);
             atomicAdd(&A.stats[10], f);
+            atomicAdd(&A.stats[11], sl);
             atomicAdd(&A.stats[0], a);
             atomicAdd(&A.stats[1], b);
             atomicAdd(&A.stats[2], c);

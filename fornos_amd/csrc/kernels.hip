@@ -105,6 +105,7 @@ struct fornos_solver {
     float* out = nullptr;
     uint32_t* counts = nullptr;
     unsigned long long* stats = nullptr;  // kStats counters of the counting run
+    uint32_t* fbflags = nullptr;          // per-texel fallback flags of the counting run
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
@@ -126,6 +127,7 @@ struct fornos_solver {
         if (owner) owner->partner = nullptr;
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
         dev_free(samples); dev_free(ray_samples); dev_free(out); dev_free(counts); dev_free(stats);
+        dev_free(fbflags);
         dev_free(raybuf); dev_free(counter);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (done_ev) (void)hipEventDestroy(done_ev);
@@ -175,6 +177,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.tval = KIND == kThick ? (float*)s->raybuf : nullptr;
     A.counter = s->counter;
     A.stats = s->stats;
+    A.fb = count ? s->fbflags : nullptr;
     A.S = S;
     A.P = s->P;
     A.mwords = mwords;
@@ -246,6 +249,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
     A.tval = nullptr;
     A.counter = s->counter;
     A.stats = s->stats;
+    A.fb = nullptr;
     A.S = S;
     A.P = s->P;
     A.mwords = mwords;
@@ -1090,6 +1094,12 @@ fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64
 {
     if (!s) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: null handle");
     if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_count_work: not a hemisphere solver");
+    if (!s->fbflags && s->mm->n) {
+        if (fornos_status st = dev_alloc(&s->fbflags, s->mm->n)) return st;
+    }
+    if (s->mm->n)
+        hipLaunchKernelGGL(k_fill, dim3(div_up(s->mm->n, kBlock)), dim3(kBlock), 0, s->mm->sc->stream, s->fbflags,
+                           (size_t)s->mm->n, 0u);
     if (fornos_status st = solver_launch(s, 0, s->mm->n, true)) return st;
     unsigned long long h[kStats] = {};
     HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
@@ -1108,6 +1118,17 @@ fornos_status fornos_solver_work_counters(const fornos_solver* s, uint64_t* coun
     if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_work_counters: not a hemisphere solver");
     HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
     HIP_TRY(hipMemcpy(counters, s->stats, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_fallback_flags(const fornos_solver* s, uint32_t* flags)
+{
+    if (!s || !flags) return set_error(FORNOS_INVALID_ARG, "fornos_solver_fallback_flags: null argument");
+    if (!s->hemi()) return set_error(FORNOS_INVALID_ARG, "fornos_solver_fallback_flags: not a hemisphere solver");
+    if (!s->mm->n) return FORNOS_OK;
+    if (!s->fbflags) return set_error(FORNOS_INVALID_ARG, "fornos_solver_fallback_flags: no counting run yet (fornos_solver_count_work)");
+    HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
+    HIP_TRY(hipMemcpy(flags, s->fbflags, sizeof(uint32_t) * s->mm->n, hipMemcpyDeviceToHost));
     return FORNOS_OK;
 }
 

@@ -254,10 +254,19 @@ fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
 /* All counters of the last fornos_solver_count_work run: box tests, triangle
  * tests, rays, stack overflows; then, summed over waves: walk steps, lanes
  * walking, leaf steps, lanes in leaf steps, refills, lanes refilled (their
- * ratios are the SIMD lane utilisation of each phase); then rays traced on
- * the reference tree (directions with a component below 2^-40, and
- * thickness rays redone there).  n <= 11. */
+ * ratios are the SIMD lane utilisation of each phase); then thickness rays
+ * redone on the reference tree (a hit whose leaf the reference would prune
+ * with the hit's own bound), and rays traced there from the start (a
+ * direction component below 2^-40).  n <= 12. */
 fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *counters, uint32_t n);
+/* Per-texel flags of the last fornos_solver_count_work run, count uint32
+ * entries: bit 0 = a ray of the texel overflowed the traversal stack and was
+ * redone on the stackless binary culling tree, bit 1 = a thickness ray was
+ * redone on the reference tree, bit 2 = a ray had a direction component below
+ * 2^-40 and was traced on the reference tree.  These are the paths that take
+ * rays off the 4-wide tree; tests point the oracle at the flagged texels.
+ * No reference counterpart (the reference has one traversal). */
+fornos_status fornos_solver_fallback_flags(const fornos_solver *s, uint32_t *flags);
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
  * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
  * stack entries per lane (16, 24 or 32; a ray that would overflow it is

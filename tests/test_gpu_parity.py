@@ -11,41 +11,9 @@ import pytest
 
 import fornos_amd as fa
 import oracle as orc
+from bakes import MISS, Bake, same_bits
 
 pytestmark = pytest.mark.gpu
-
-MISS = 0xFFFFFFFF
-
-
-def bits(a):
-    return np.ascontiguousarray(a, np.float32).view(np.uint32)
-
-
-def same_bits(a, b):
-    """Bitwise float equality, with every NaN equal to every NaN."""
-    a = np.ascontiguousarray(a, np.float32)
-    b = np.ascontiguousarray(b, np.float32)
-    nan = np.isnan(a) & np.isnan(b)
-    return np.array_equal(bits(a)[~nan], bits(b)[~nan]) and np.array_equal(np.isnan(a), np.isnan(b))
-
-
-class Bake:
-    """A prepared workload with its oracle-side BVH."""
-
-    def __init__(self, cfg, cull=True):
-        self.prep = fa.prepare(cfg)
-        self.cfg = self.prep["cfg"]
-        self.tex = self.prep["texels"].arrays()
-        self.scene = self.prep["scene"]
-        hi = self.prep["high"].arrays()
-        self.nodes, self.spos, self.snrm, self.stri = orc.bvh_build(hi)
-        self.mm = fa.MeshMapping().init(self.prep["texels"], self.scene, cull)
-        self.ms = self.mm.run_all()
-        self.coords, self.tidx = self.mm.read()
-
-    def oracle_map(self, begin=0, end=None, cull=True):
-        return orc.meshmapping(self.tex["positions"], self.tex["directions"], self.spos, self.nodes, cull,
-                               begin=begin, end=end)
 
 
 @pytest.fixture(scope="module")
