@@ -139,7 +139,44 @@ struct fornos_solver {
     }
 };
 
+// Scratch of fornos_solver_run_all_noalloc, the entry point that allocates
+// nothing: the k_rays work counter in the first kScratchHead bytes, then one
+// launch's bent-normal hit bits or thickness distances.  Each device holds its
+// own copy at its own address for the life of the process; host code reaches
+// it only through the address hipGetSymbolAddress returns for that device
+// (scratch_for), never through the array's name.
+constexpr size_t kScratchBytes = 64ull << 20, kScratchHead = 256, kScratchData = kScratchBytes - kScratchHead;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kScratchDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current.
+fornos_status scratch_for(int device, uint8_t** out)
+{
+    if (device < 0 || device >= kScratchDevices)
+        return set_error(FORNOS_UNSUPPORTED, "fornos_solver_run_all_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        HIP_TRY(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        const hipError_t e2 = hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return set_error(FORNOS_HIP_ERROR, std::string("fornos_solver_run_all_noalloc: hipGetSymbolAddress(scratch): ") +
+                                                   hipGetErrorString(e));
+        HIP_TRY(e2);
+        g_scratch[device] = (uint8_t*)p;
+    }
+    *out = g_scratch[device];
+    return FORNOS_OK;
+}
 
 // Launch a hemisphere kernel over texels [t0, t1).
 // k_rays with a stack of 16, 24 or 32 entries per lane
@@ -160,8 +197,24 @@ bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
            (stack == 16 || stack == 24 || stack == 32);
 }
 
+// Device buffers a hemisphere pass writes: the solver's own, or the caller's
+// outputs with scratch from the library's fixed array (the no-alloc entry).
+struct PassBufs {
+    float* out;              // per-texel results (global texel index)
+    uint32_t* counts;        // per-texel ray counts (global texel index)
+    void* raybuf;            // one launch's BN hit bits or thickness distances
+    unsigned int* counter;   // k_rays work counter
+    uint32_t chunk_texels;   // texels per k_rays launch (raybuf holds one launch)
+    bool timed;              // record per-launch events into s->events
+};
+
+PassBufs own_bufs(const fornos_solver* s)
+{
+    return {s->out, s->counts, s->raybuf, s->counter, s->chunk_texels, s->timed};
+}
+
 template <int KIND>
-fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count)
+fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count, const PassBufs& B)
 {
     fornos_meshmap* mm = s->mm;
     fornos_scene* sc = mm->sc;
@@ -172,10 +225,10 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.tidx = mm->tidx;
     A.samples = s->ray_samples;
     const uint32_t mwords = (S + 31) / 32;
-    A.cnt = s->counts;
-    A.mask = KIND == kBN ? (uint32_t*)s->raybuf : nullptr;
-    A.tval = KIND == kThick ? (float*)s->raybuf : nullptr;
-    A.counter = s->counter;
+    A.cnt = B.counts;
+    A.mask = KIND == kBN ? (uint32_t*)B.raybuf : nullptr;
+    A.tval = KIND == kThick ? (float*)B.raybuf : nullptr;
+    A.counter = B.counter;
     A.stats = s->stats;
     A.fb = count ? s->fbflags : nullptr;
     A.S = S;
@@ -192,8 +245,8 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     R.pixt = mm->pixt;
     R.mask = A.mask;
     R.tval = A.tval;
-    R.out = s->out;
-    R.counts = s->counts;
+    R.out = B.out;
+    R.counts = B.counts;
     R.S = S;
     R.P = s->P;
     R.mwords = mwords;
@@ -203,17 +256,17 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         hipLaunchKernelGGL(k_fill, dim3(div_up(n, kBlock)), dim3(kBlock), 0, sc->stream, (uint32_t*)p, n, v);
     };
     if (count) fill(s->stats, 2 * kStats, 0);
-    for (uint32_t c0 = t0; c0 < t1; c0 += s->chunk_texels) {
-        const uint32_t nt = std::min(s->chunk_texels, t1 - c0);
+    for (uint32_t c0 = t0; c0 < t1; c0 += B.chunk_texels) {
+        const uint32_t nt = std::min(B.chunk_texels, t1 - c0);
         A.t0 = R.t0 = c0;
         A.ntex = R.ntex = nt;
         const uint32_t blocks = std::min<uint32_t>(sc->cus * s->blocks_per_cu, div_up((uint64_t)nt * S, kBlock));
-        fill(s->counter, 1, 0);
-        if (KIND == kAO) fill(s->counts + c0, nt, 0);
-        if (KIND == kBN) fill(s->raybuf, (size_t)nt * mwords, 0);
-        if (KIND == kThick) fill(s->raybuf, (size_t)nt * S, 0x7f7fffffu);  // FLT_MAX
+        fill(B.counter, 1, 0);
+        if (KIND == kAO) fill(B.counts + c0, nt, 0);
+        if (KIND == kBN) fill(B.raybuf, (size_t)nt * mwords, 0);
+        if (KIND == kThick) fill(B.raybuf, (size_t)nt * S, 0x7f7fffffu);  // FLT_MAX
         hipEvent_t* ev = nullptr;
-        if (s->timed) {
+        if (B.timed) {
             s->events.resize(s->events.size() + 3, nullptr);
             ev = &s->events[s->events.size() - 3];
             for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
@@ -345,9 +398,9 @@ fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool cou
         hipLaunchKernelGGL(k_attr<true>, dim3(blocks), dim3(kBlock), 0, sc->stream, mm->coords, mm->tidx,
                            sc->snrm, mm->pixt, t0, t1, s->p.tangent_space, s->out);
         break;
-    case FORNOS_SOLVER_AO: return launch_hemi<kAO>(s, t0, t1, count);
-    case FORNOS_SOLVER_BENT_NORMALS: return launch_hemi<kBN>(s, t0, t1, count);
-    case FORNOS_SOLVER_THICKNESS: return launch_hemi<kThick>(s, t0, t1, count);
+    case FORNOS_SOLVER_AO: return launch_hemi<kAO>(s, t0, t1, count, own_bufs(s));
+    case FORNOS_SOLVER_BENT_NORMALS: return launch_hemi<kBN>(s, t0, t1, count, own_bufs(s));
+    case FORNOS_SOLVER_THICKNESS: return launch_hemi<kThick>(s, t0, t1, count, own_bufs(s));
     }
     HIP_TRY(hipGetLastError());
     return FORNOS_OK;
@@ -988,6 +1041,34 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
         s->rays_ms += a;
         s->reduce_ms += b;
     }
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_run_all_noalloc(fornos_solver* s, float* out, uint32_t* counts)
+{
+    if (!s || !out || !counts) return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_all_noalloc: null argument");
+    if (!s->hemi())
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_all_noalloc: not a hemisphere solver (AO, bent normals, thickness)");
+    const uint32_t n = s->mm->n, S = s->p.sample_count;
+    // one texel's share of a launch's per-ray results: thickness distances or bent-normal hit bits
+    const size_t per_texel = s->kind == FORNOS_SOLVER_THICKNESS ? 4ull * S
+                             : s->kind == FORNOS_SOLVER_BENT_NORMALS ? 4ull * ((S + 31) / 32) : 0;
+    if (per_texel > kScratchData)
+        return set_error(FORNOS_UNSUPPORTED, "fornos_solver_run_all_noalloc: one texel's per-ray results exceed the "
+                                             "64 MiB scratch array");
+    if (n == 0) return FORNOS_OK;
+    uint8_t* base = nullptr;
+    if (fornos_status st = scratch_for(s->mm->sc->device, &base)) return st;
+    PassBufs B{out, counts, base + kScratchHead, (unsigned int*)base, s->chunk_texels, false};
+    if (per_texel) B.chunk_texels = (uint32_t)std::min<size_t>(B.chunk_texels, kScratchData / per_texel);
+    fornos_status st = FORNOS_OK;
+    switch (s->kind) {
+    case FORNOS_SOLVER_AO: st = launch_hemi<kAO>(s, 0, n, false, B); break;
+    case FORNOS_SOLVER_BENT_NORMALS: st = launch_hemi<kBN>(s, 0, n, false, B); break;
+    default: st = launch_hemi<kThick>(s, 0, n, false, B); break;
+    }
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
     return FORNOS_OK;
 }
 

@@ -247,6 +247,25 @@ fornos_status fornos_solver_read_counts(const fornos_solver *s, uint32_t *out);
  * events on the scene's stream; *launches = number of k_rays launches. */
 fornos_status fornos_solver_last_timing(const fornos_solver *s, float *rays_ms, float *reduce_ms,
                                         uint32_t *launches);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The pass of fornos_solver_run_all for a hemisphere
+ * solver (AO, bent normals, thickness), with its results written into
+ * caller-owned DEVICE memory on the solver's device -- out: count * (1 or 3)
+ * floats, as fornos_solver_read returns them; counts: count uint32, as
+ * fornos_solver_read_counts -- and its scratch (the ray counter and one
+ * launch's bent-normal hit bits or thickness distances) in a fixed 64 MiB
+ * device array of the library instead of memory allocated for the call.
+ * Calls no hipMalloc or hipFree and waits for the pass to finish.  Launches
+ * are cut to fit the array (thickness: 16383 texels per launch at
+ * sample_count 1024), so a pass of any texel count runs; a solver of which one
+ * texel alone needs more than the array is rejected with FORNOS_UNSUPPORTED.
+ * Each device holds one such array for the life of the process and every call
+ * on that device shares it: calls that overlap on one device (from several
+ * threads, or on solvers of different scenes) race and give wrong results, so
+ * the caller serialises them.  The solver's own results, progress and pairing
+ * are left as they are (a paired bent-normals solver is not computed). */
+fornos_status fornos_solver_run_all_noalloc(fornos_solver *s, float *out, uint32_t *counts);
 /* Traversal work of the last run_all of a hemisphere solver, counted by an
  * instrumented run of the same kernel: node tests, triangle tests, rays. */
 fornos_status fornos_solver_count_work(fornos_solver *s, uint64_t *node_tests,
