@@ -143,6 +143,7 @@ _SIGS = [
                                         C.POINTER(C.c_uint64)]),
     ("fornos_solver_work_counters", None, [H, P, C.c_uint32]),
     ("fornos_solver_run_all_noalloc", None, [H, P, P]),
+    ("fornos_solver_run_texels", None, [H, P, C.c_uint32, C.POINTER(C.c_float)]),
     ("fornos_solver_fallback_flags", None, [H, P]),
     ("fornos_solver_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("fornos_solver_get_launch", None, [H, u32p, u32p, u32p]),

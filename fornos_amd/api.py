@@ -333,6 +333,14 @@ class Solver(_Handle):
         check(lib().fornos_solver_progress(self._h, C.byref(p)))
         return p.value
 
+    def run_texels(self, texels):
+        """The pass for the listed texels only (increasing global indices);
+        their results equal run_all's (fornos_solver_run_texels)."""
+        t = np.ascontiguousarray(texels, np.uint32)
+        ms = C.c_float()
+        check(lib().fornos_solver_run_texels(self._h, ptr(t), len(t), C.byref(ms)))
+        return ms.value
+
     def reset(self):
         check(lib().fornos_solver_reset(self._h))
 

@@ -73,10 +73,18 @@ struct RayArgs {
     unsigned int* __restrict__ counter;  // next ray of the launch to hand out
     unsigned long long* __restrict__ stats;  // kStats counters (COUNT only)
     uint32_t* __restrict__ fb;          // kFb* flags per texel, global texel index (COUNT only)
-    uint32_t S, P, t0, ntex, mwords;    // texels [t0, t0 + ntex), S rays each
+    const uint32_t* __restrict__ list;  // null: texels [t0, t0 + ntex); else list[t0 .. t0 + ntex)
+    uint32_t S, P, t0, ntex, mwords;    // ntex texels, S rays each
     float mind, maxd;
     uint32_t refill;                    // refill a wave once this many lanes are idle
 };
+
+// Global index of the launch's q-th texel (the list is wave-uniform, so the
+// contiguous case costs one scalar branch).
+__device__ inline uint32_t texel_of(const uint32_t* list, uint32_t t0, uint32_t q)
+{
+    return list ? list[t0 + q] : t0 + q;
+}
 
 struct Lane {
     Ray ray;
@@ -94,11 +102,11 @@ template <int KIND>
 __device__ inline void finish(const RayArgs& A, Lane& L, bool hit)
 {
     if (KIND == kAO) {
-        if (hit) atomicAdd(&A.cnt[A.t0 + L.q], 1u);
+        if (hit) atomicAdd(&A.cnt[texel_of(A.list, A.t0, L.q)], 1u);
     } else if (KIND == kBN) {
         if (hit) atomicOr(&A.mask[(size_t)L.q * A.mwords + (L.s >> 5)], 1u << (L.s & 31));
     } else if (KIND == kAOBN) {
-        if (hit) atomicAdd(&A.cnt[A.t0 + L.q], 1u);
+        if (hit) atomicAdd(&A.cnt[texel_of(A.list, A.t0, L.q)], 1u);
         if (hit || L.bn) atomicOr(&A.mask[(size_t)L.q * A.mwords + (L.s >> 5)], 1u << (L.s & 31));
     } else {
         if (L.best != FLT_MAX) A.tval[(size_t)L.q * A.S + L.s] = L.best;
@@ -243,7 +251,7 @@ __device__ inline void wide_step(const RayArgs& A, Lane& L, uint32_t* stk_node, 
     if (L.sp + n > STK) {  // redo the ray on the stackless binary culling tree
         if (COUNT) {
             ++nov;
-            atomicOr(&A.fb[A.t0 + L.q], kFbOverflow);
+            atomicOr(&A.fb[texel_of(A.list, A.t0, L.q)], kFbOverflow);
         }
         L.slot = kNoChild;
         L.mode = kCullTree;
@@ -299,7 +307,7 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
                 if (r == 2) {
                     if (COUNT) {
                         ++nref;
-                        atomicOr(&A.fb[A.t0 + L.q], kFbRedo);
+                        atomicOr(&A.fb[texel_of(A.list, A.t0, L.q)], kFbRedo);
                     }
                     L.mode = kRefTree;
                     L.node = 0;
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
             if (!L.active) {
                 const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
                 if (r < total) {
-                    const uint32_t q = r / A.S, s = r - q * A.S, g = A.t0 + q;
+                    const uint32_t q = r / A.S, s = r - q * A.S, g = texel_of(A.list, A.t0, q);
                     const uint32_t t = A.tidx[g];
                     if (t != kMiss) {
                         const Basis B = ray_basis(A.sc, A.coords[g], t);
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 } else if (r == 2) {
                     if (COUNT) {
                         ++nref;
-                        atomicOr(&A.fb[A.t0 + L.q], kFbRedo);
+                        atomicOr(&A.fb[texel_of(A.list, A.t0, L.q)], kFbRedo);
                     }
                     L.mode = kRefTree;
                     L.node = 0;
@@ -509,6 +517,7 @@ struct ReduceArgs {
     const float* __restrict__ tval;
     float* __restrict__ out;            // 1 or 3 floats per texel (global texel index)
     uint32_t* __restrict__ counts;      // per texel (AO: already counted by k_rays)
+    const uint32_t* __restrict__ list;  // as RayArgs::list
     uint32_t S, P, t0, ntex, mwords;
     float maxd;
     int tangent;
@@ -522,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
 {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= A.ntex) return;
-    const uint32_t g = A.t0 + q, S = A.S;
+    const uint32_t g = texel_of(A.list, A.t0, q), S = A.S;
     const uint32_t t = A.tidx[g];
     if (KIND == kAO) {
         // the reference's sequential float sum of 0/1 values is the exact count
@@ -577,6 +586,13 @@ __global__ void k_fill(uint32_t* p, size_t n, uint32_t v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// p[list[i]] = v for the n listed words.
+__global__ void k_fill_list(uint32_t* p, const uint32_t* list, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[list[i]] = v;
 }
 
 }  // namespace

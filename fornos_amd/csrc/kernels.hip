@@ -106,6 +106,8 @@ struct fornos_solver {
     uint32_t* counts = nullptr;
     unsigned long long* stats = nullptr;  // kStats counters of the counting run
     uint32_t* fbflags = nullptr;          // per-texel fallback flags of the counting run
+    uint32_t* dlist = nullptr;            // texel list of the last fornos_solver_run_texels
+    uint32_t dlist_cap = 0;
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
@@ -127,7 +129,7 @@ struct fornos_solver {
         if (owner) owner->partner = nullptr;
         if (mm && mm->sc && mm->sc->stream) (void)hipStreamSynchronize(mm->sc->stream);
         dev_free(samples); dev_free(ray_samples); dev_free(out); dev_free(counts); dev_free(stats);
-        dev_free(fbflags);
+        dev_free(fbflags); dev_free(dlist);
         dev_free(raybuf); dev_free(counter);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (done_ev) (void)hipEventDestroy(done_ev);
@@ -213,8 +215,11 @@ PassBufs own_bufs(const fornos_solver* s)
     return {s->out, s->counts, s->raybuf, s->counter, s->chunk_texels, s->timed};
 }
 
+// Launch a hemisphere pass over texels [t0, t1), or, with a device `list` of
+// global texel indices, over list[t0 .. t1).
 template <int KIND>
-fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count, const PassBufs& B)
+fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count, const PassBufs& B,
+                          const uint32_t* list = nullptr)
 {
     fornos_meshmap* mm = s->mm;
     fornos_scene* sc = mm->sc;
@@ -231,6 +236,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.counter = B.counter;
     A.stats = s->stats;
     A.fb = count ? s->fbflags : nullptr;
+    A.list = list;
     A.S = S;
     A.P = s->P;
     A.mwords = mwords;
@@ -247,6 +253,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     R.tval = A.tval;
     R.out = B.out;
     R.counts = B.counts;
+    R.list = list;
     R.S = S;
     R.P = s->P;
     R.mwords = mwords;
@@ -262,7 +269,10 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         A.ntex = R.ntex = nt;
         const uint32_t blocks = std::min<uint32_t>(sc->cus * s->blocks_per_cu, div_up((uint64_t)nt * S, kBlock));
         fill(B.counter, 1, 0);
-        if (KIND == kAO) fill(B.counts + c0, nt, 0);
+        if (KIND == kAO && list)
+            hipLaunchKernelGGL(k_fill_list, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, B.counts, list + c0, nt, 0u);
+        else if (KIND == kAO)
+            fill(B.counts + c0, nt, 0);
         if (KIND == kBN) fill(B.raybuf, (size_t)nt * mwords, 0);
         if (KIND == kThick) fill(B.raybuf, (size_t)nt * S, 0x7f7fffffu);  // FLT_MAX
         hipEvent_t* ev = nullptr;
@@ -284,9 +294,9 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     return FORNOS_OK;
 }
 
-// AO solver `s` and its BN partner over texels [t0, t1) with one traversal
-// (k_rays<kAOBN>), then both reductions.
-fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
+// AO solver `s` and its BN partner over texels [t0, t1) (or list[t0 .. t1))
+// with one traversal (k_rays<kAOBN>), then both reductions.
+fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint32_t* list = nullptr)
 {
     fornos_solver* b = s->partner;
     fornos_meshmap* mm = s->mm;
@@ -303,6 +313,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
     A.counter = s->counter;
     A.stats = s->stats;
     A.fb = nullptr;
+    A.list = list;
     A.S = S;
     A.P = s->P;
     A.mwords = mwords;
@@ -324,6 +335,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
     Ra.mwords = mwords;
     Ra.maxd = s->p.max_distance;
     Ra.tangent = 0;
+    Ra.list = list;
     ReduceArgs Rb = Ra;
     Rb.samples = b->samples;
     Rb.mask = A.mask;
@@ -339,7 +351,10 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1)
         A.ntex = Ra.ntex = Rb.ntex = nt;
         const uint32_t blocks = std::min<uint32_t>(sc->cus * s->blocks_per_cu, div_up((uint64_t)nt * S, kBlock));
         fill(s->counter, 1, 0);
-        fill(s->counts + c0, nt, 0);
+        if (list)
+            hipLaunchKernelGGL(k_fill_list, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, s->counts, list + c0, nt, 0u);
+        else
+            fill(s->counts + c0, nt, 0);
         fill(b->raybuf, (size_t)nt * mwords, 0);
         hipEvent_t* ev = nullptr;
         if (s->timed) {
@@ -1042,6 +1057,48 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
         s->reduce_ms += b;
     }
     return FORNOS_OK;
+}
+
+fornos_status fornos_solver_run_texels(fornos_solver* s, const uint32_t* texels, uint32_t n, float* ms)
+{
+    if (!s || (!texels && n)) return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: null argument");
+    if (!s->hemi())
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: not a hemisphere solver (AO, bent normals, thickness)");
+    const uint32_t count = s->mm->n;
+    for (uint32_t i = 0; i < n; ++i)
+        if (texels[i] >= count || (i && texels[i] <= texels[i - 1]))
+            return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: texel indices must be increasing and below the "
+                                                 "texel count");
+    if (s->owner) return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: a paired bent-normals solver runs through its AO solver");
+    hipStream_t st = s->mm->sc->stream;
+    if (n > s->dlist_cap) {
+        HIP_TRY(hipStreamSynchronize(st));
+        dev_free(s->dlist);
+        s->dlist = nullptr;
+        s->dlist_cap = 0;
+        if (fornos_status e = dev_alloc(&s->dlist, n)) return e;
+        s->dlist_cap = n;
+    }
+    Timer tm(st);
+    if (fornos_status e = tm.start()) return e;
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(s->dlist, texels, sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+        fornos_status e = FORNOS_OK;
+        if (s->partner) {
+            e = launch_pair(s, 0, n, s->dlist);
+        } else {
+            switch (s->kind) {
+            case FORNOS_SOLVER_AO: e = launch_hemi<kAO>(s, 0, n, false, own_bufs(s), s->dlist); break;
+            case FORNOS_SOLVER_BENT_NORMALS: e = launch_hemi<kBN>(s, 0, n, false, own_bufs(s), s->dlist); break;
+            default: e = launch_hemi<kThick>(s, 0, n, false, own_bufs(s), s->dlist); break;
+            }
+        }
+        if (e) return e;
+    }
+    if (fornos_status e = mark_done(s)) return e;
+    if (s->partner)
+        if (fornos_status e = mark_done(s->partner)) return e;
+    return tm.stop(ms);
 }
 
 fornos_status fornos_solver_run_all_noalloc(fornos_solver* s, float* out, uint32_t* counts)

@@ -7,7 +7,8 @@ force only (Src/solver_ao.cpp, Src/solver_bentnormals.cpp).  The denoised
 bake runs the plain GPU baker at 16 samples plus the position, normal and
 mapping-distance passes, feeds them through the network, and falls back to
 the plain 1024-sample bake on every 32 x 32 block whose output fails a
-consistency check against its noisy input.
+consistency check against its noisy input (those texels are then exactly the
+plain bake's).
 
 Training pairs are 16-sample inputs and 1024-sample targets baked by the plain
 GPU method on windows of seeded procedural meshes (the sphere, heightfield and
@@ -33,6 +34,7 @@ from .api import (AmbientOcclusionSolver, BentNormalsSolver, HeightSolver, Mesh,
 
 IN, OUT, MASK = 12, 4, 11
 LOW_S, HIGH_S = 16, 1024
+PLAIN_SEED = 7  # sample-table seed of the plain 1024-sample bake (and so of the fallback)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "denoiser_l4c32.npz")
 
 
@@ -228,7 +230,7 @@ def readback_buffers(n):
             torch.empty((n, 3), dtype=torch.float32, pin_memory=True))
 
 
-def plain_bake(job, S=HIGH_S, seed=7, host=None):
+def plain_bake(job, S=HIGH_S, seed=PLAIN_SEED, host=None):
     """Mapping + AO + bent normals at S samples; host results and wall time.
     host: readback_buffers to read into (default: new arrays)."""
     t0 = time.perf_counter()
@@ -245,10 +247,12 @@ def plain_bake(job, S=HIGH_S, seed=7, host=None):
     return a, b, t, mm
 
 
-def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True, host=None):
+def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True, host=None, masks=False):
     """Mapping + 16-sample AO / bent normals + G-buffers -> network -> per-tile
     check -> plain 1024-sample bake of the failing tiles.  Host results, wall
-    time and a breakdown.  host: readback_buffers to read into."""
+    time and a breakdown.  host: readback_buffers to read into.  masks: also
+    return the failing blocks ("fail_blocks", [tiles, tiles] bool) and the
+    texels that fell back ("fallback_mask", per texel) in the breakdown."""
     torch = _torch()
     W = job.size
     t0 = time.perf_counter()
@@ -280,20 +284,25 @@ def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True,
     t3 = time.perf_counter()
     n_fb = 0
     if fallback and bad.any():
+        # the plain bake itself on the failing blocks' texels: same mapping,
+        # same table, each texel its own sample permutation, so these texels
+        # equal plain_bake's bit for bit
         idx = job.texels.a["indices"].astype(np.int64)
         sel = bad[(idx // W) // tile, (idx % W) // tile]
         n_fb = int(sel.sum())
-        sub = job.texels.subset(sel)
-        fmm = job.mapping(sub)
-        fao, fbn = job.solvers(fmm, HIGH_S, seed + 1)
-        fao.run_all()
-        fbn.run_all()
-        a[sel] = fao.getResults()
-        b[sel] = fbn.getResults()
+        fao, fbn = job.solvers(mm, HIGH_S, PLAIN_SEED)
+        fao.run_texels(np.flatnonzero(sel))
+        a[sel] = fao.getResults()[sel]
+        b[sel] = fbn.getResults()[sel]
     t4 = time.perf_counter()
-    return a, b, t4 - t0, {"bake16_s": t1 - t0, "network_s": t2 - t1, "network_ms": net_ms,
-                           "check_s": t3 - t2, "fallback_s": t4 - t3, "fallback_tiles": int(bad.sum()),
-                           "tiles": int(nt * nt), "fallback_texels": n_fb}
+    info = {"bake16_s": t1 - t0, "network_s": t2 - t1, "network_ms": net_ms, "check_s": t3 - t2,
+            "fallback_s": t4 - t3, "fallback_tiles": int(bad.sum()), "tiles": int(nt * nt), "fallback_texels": n_fb}
+    if masks:
+        idx = job.texels.a["indices"].astype(np.int64)
+        info["fail_blocks"] = bad
+        info["fallback_mask"] = (bad[(idx // W) // tile, (idx % W) // tile] if fallback
+                                 else np.zeros(len(idx), bool))
+    return a, b, t4 - t0, info
 
 
 def errors(ao, bn, ref_ao, ref_bn, hit):

@@ -247,6 +247,18 @@ fornos_status fornos_solver_read_counts(const fornos_solver *s, uint32_t *out);
  * events on the scene's stream; *launches = number of k_rays launches. */
 fornos_status fornos_solver_last_timing(const fornos_solver *s, float *rays_ms, float *reduce_ms,
                                         uint32_t *launches);
+/* The pass of a hemisphere solver (AO, bent normals, thickness) for the
+ * listed texels only: n global texel indices into the solver's texel set,
+ * strictly increasing.  Their results, read with fornos_solver_read /
+ * fornos_solver_read_counts, equal those of fornos_solver_run_all bit for bit
+ * (each texel keeps its own sample permutation, texel % n_perm, as in
+ * Shaders/ao_step1.comp:153); other texels' results are left as they are.
+ * An AO solver paired with a bent-normals solver computes both.  Progress is
+ * unchanged; *ms (may be NULL) = device time.  No reference counterpart (the
+ * reference re-bakes a region only by baking the whole map,
+ * Src/solver_ao.cpp:81-131); the denoised bake's per-tile fallback to the
+ * plain bake uses it. */
+fornos_status fornos_solver_run_texels(fornos_solver *s, const uint32_t *texels, uint32_t n, float *ms);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The pass of fornos_solver_run_all for a hemisphere
