@@ -35,32 +35,51 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        u32, i32, f32 = C.c_uint32, C.c_int, C.c_float
-        sigs = {
-            "orc_vertex_normals": (None, [P, u32, P, u32, P, u32, P, P]),
-            "orc_vertex_normals_aggressive": (u32, [P, P, u32, P, u32, P, P]),
-            "orc_face_normals": (None, [P, P, u32, P, u32, P, P]),
-            "orc_tangent_space": (None, [P, P, P, P, u32, P, u32, P, P]),
-            "orc_mapuv": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, P, P, P, P, P]),
-            "orc_compress": (u32, [P, u32, P]),
-            "orc_mapuv_edge": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, f32, P, P, P, P, P]),
-            "orc_samples": (None, [u32, u32, P, P]),
-            "orc_bvh_build": (u32, [P, u32, P, P, P, u32, u32, u32, P, P, P, P, P]),
-            "orc_meshmapping": (None, [P, P, u32, u32, P, P, u32, i32, P, P, i32]),
-            "orc_hemisphere": (None, [i32, u32, u32, f32, f32, P, P, P, u32, u32, P, P, P, u32, P, P, P, i32]),
-            "orc_tangentspace": (None, [P, P, P, u32, u32, P]),
-            "orc_height": (None, [P, u32, P]),
-            "orc_texel_attr": (None, [i32, P, P, u32, P, P]),
-            "orc_map_bruteforce": (None, [P, P, u32, P, u32, i32, P, P]),
-            "orc_hemi_bruteforce": (f32, [P, P, f32, P, u32]),
-        }
-        for name, (res, args) in sigs.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+_mutants = {}
+
+
+def mutant(name):
+    """The oracle library with one planted mistake ("bn_maxd": bent normals
+    count only hits below maxD; "thick_clamp": thickness clamps hits to maxD),
+    for tests that show the pins reject it."""
+    if name not in _mutants:
+        path = os.path.join(HERE, "_build", "liborc_mut_%s.so" % name)
+        if not os.path.exists(path):
+            build()
+        _mutants[name] = _load(path)
+    return _mutants[name]
+
+
+def _load(path):
+    L = C.CDLL(path)
+    u32, i32, f32 = C.c_uint32, C.c_int, C.c_float
+    sigs = {
+        "orc_vertex_normals": (None, [P, u32, P, u32, P, u32, P, P]),
+        "orc_vertex_normals_aggressive": (u32, [P, P, u32, P, u32, P, P]),
+        "orc_face_normals": (None, [P, P, u32, P, u32, P, P]),
+        "orc_tangent_space": (None, [P, P, P, P, u32, P, u32, P, P]),
+        "orc_mapuv": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, P, P, P, P, P]),
+        "orc_compress": (u32, [P, u32, P]),
+        "orc_mapuv_edge": (i32, [P, P, P, P, P, u32, P, P, P, P, u32, u32, f32, P, P, P, P, P]),
+        "orc_samples": (None, [u32, u32, P, P]),
+        "orc_bvh_build": (u32, [P, u32, P, P, P, u32, u32, u32, P, P, P, P, P]),
+        "orc_meshmapping": (None, [P, P, u32, u32, P, P, u32, i32, P, P, i32]),
+        "orc_hemisphere": (None, [i32, u32, u32, f32, f32, P, P, P, u32, u32, P, P, P, u32, P, P, P, i32]),
+        "orc_tangentspace": (None, [P, P, P, u32, u32, P]),
+        "orc_height": (None, [P, u32, P]),
+        "orc_texel_attr": (None, [i32, P, P, u32, P, P]),
+        "orc_map_bruteforce": (None, [P, P, u32, P, u32, i32, P, P]),
+        "orc_hemi_bruteforce": (f32, [P, P, f32, P, u32]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _p(a):
@@ -185,8 +204,9 @@ KIND_AO, KIND_BN, KIND_THICK = 0, 1, 2
 
 
 def hemisphere(kind, S, mind, maxd, table, coords, tidx, spos, snrm, nodes, begin=0, end=None,
-               nthreads=0, stats=False):
-    """Per-texel AO / bent normal (pre tangent space) / thickness over texels [begin, end)."""
+               nthreads=0, stats=False, library=None):
+    """Per-texel AO / bent normal (pre tangent space) / thickness over texels
+    [begin, end).  library: another build of the oracle (a mutant)."""
     coords, tidx = _c(coords, np.float32), _c(tidx, np.uint32)
     table = _c(table, np.float32)
     end = len(tidx) if end is None else end
@@ -196,7 +216,7 @@ def hemisphere(kind, S, mind, maxd, table, coords, tidx, spos, snrm, nodes, begi
     st = np.zeros(2, np.uint64)
     nodes = _c(nodes, np.uint32)
     Pn = len(table) // S
-    lib().orc_hemisphere(kind, S, Pn, float(mind), float(maxd), _p(table), _p(coords), _p(tidx), begin, end,
+    (library or lib()).orc_hemisphere(kind, S, Pn, float(mind), float(maxd), _p(table), _p(coords), _p(tidx), begin, end,
                          _p(_c(spos, np.float32)), _p(_c(snrm, np.float32)), _p(nodes), len(nodes),
                          _p(out), _p(counts), _p(st) if stats else None, nthreads)
     if stats:

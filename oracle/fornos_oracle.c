@@ -854,8 +854,17 @@ ORC_API void orc_hemisphere(int kind, uint32_t S, uint32_t P, float mind, float 
                 float r = (t != FLT_MAX && t < maxd) ? 1.0f : 0.0f;
                 facc += r; cnt += (uint32_t)r;
             } else if (kind == 1) {          /* bentnormals_step1.comp:170 */
+#ifdef ORC_MUTANT_BN_MAXD
+                /* planted mistake (mutant builds for the pin tests only) */
+                if (t != FLT_MAX && t < maxd) ++cnt; else vacc = add3(vacc, dir);
+#else
                 if (t != FLT_MAX) ++cnt; else vacc = add3(vacc, dir);
+#endif
             } else {                         /* thick_step1.comp:158 */
+#ifdef ORC_MUTANT_THICK_CLAMP
+                /* planted mistake (mutant builds for the pin tests only) */
+                if (t != FLT_MAX && t > maxd) t = maxd;
+#endif
                 facc += (t != FLT_MAX) ? t : maxd;
                 if (t != FLT_MAX) ++cnt;
             }

@@ -50,6 +50,22 @@ def test_obj_matches_oracle(tmp_path):
     same(fa.Mesh.load(p), ref)
 
 
+def test_obj_oracle_known_answer(tmp_path):
+    # OBJ_TEXT by hand: one mesh vertex per face corner (0-based position,
+    # texcoord, normal indices), polygons fanned from their first corner
+    p = tmp_path / "m.obj"
+    p.write_text(OBJ_TEXT)
+    ref = mo.load_obj(str(p))
+    assert np.array_equal(ref["positions"], np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0.5, 2, -0.25],
+                                                        [2, 2, 2]]))
+    assert np.array_equal(ref["texcoords"], np.float32([[0, 0], [1, 0], [1, 1], [0, 1]]))
+    assert np.array_equal(ref["normals"], np.float32([[0, 0, 1]]))
+    corners = [(0, 0), (1, 1), (2, 2), (3, 3), (3, 3), (2, 2), (4, 0), (0, 0), (1, 1), (2, 2), (4, 3), (5, 0)]
+    assert np.array_equal(ref["vertices"], np.uint32([(v, t, 0) for v, t in corners]))
+    assert np.array_equal(ref["triangles"], np.uint32([[0, 1, 2], [0, 2, 3], [4, 5, 6], [7, 8, 9], [7, 9, 10],
+                                                       [7, 10, 11]]))
+
+
 @pytest.mark.parametrize("text", ["v 1 2 3\nv 1 2 3\nv 1 2 3\nf 1//1 2//1 3//1\nvn 0 0 1\n",
                                   "v 1 2 3\nv 4 5 6\nv 7 8 9\nf 1 2 3\nf 3 2 1\n"])
 def test_obj_face_forms(tmp_path, text):
@@ -123,8 +139,45 @@ def test_ply_matches_oracle(tmp_path, fmt, coord, idx, normals):
     p = tmp_path / "m.ply"
     write_ply(p, fmt, pos, faces, st=st, vn=vn, fn=fn, coord=coord, idx=idx)
     ref = mo.load_ply(str(p))
-    assert len(ref["triangles"]) == 1 + 2 + 1 + 3
+    # known answer: the arrays written, one mesh vertex per triangle corner,
+    # polygons fanned from their first corner; a face normal is indexed by
+    # its face
+    corners = [0, 1, 2, 2, 3, 4, 2, 4, 5, 5, 6, 7, 8, 7, 6, 8, 6, 5, 8, 5, 4]
+    face_of = [0, 0, 0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 3]
+    none = 0xFFFFFFFF
+    assert np.array_equal(ref["positions"], pos)
+    assert np.array_equal(ref["texcoords"], st)
+    assert np.array_equal(ref["normals"], vn if vn is not None else fn if fn is not None else np.zeros((0, 3)))
+    nidx = corners if vn is not None else face_of if fn is not None else [none] * len(corners)
+    assert np.array_equal(ref["vertices"], np.uint32([(v, v, k) for v, k in zip(corners, nidx)]))
+    assert np.array_equal(ref["triangles"], np.arange(len(corners), dtype=np.uint32).reshape(-1, 3))
     same(fa.Mesh.load(p), ref)
+
+
+def test_face_normals_oracle_closed_forms():
+    # Mesh::computeFaceNormals (Src/mesh.cpp:627-646): normalize(cross(p1 - p0, p2 - p0))
+    # per triangle, and each corner's normal index set to its triangle, the
+    # later triangle winning on a shared vertex
+    from meshes import grid
+    # (normalize is v * (1/sqrt(v.v)), Src/math.h, so a unit result can be 1 ulp short)
+    g = grid(3)
+    n, vn = orc.face_normals(g["positions"], g["vertices"], g["triangles"])
+    assert np.all(n[:, :2] == 0) and np.all(np.abs(n[:, 2] - 1) <= 6e-8)
+    assert np.array_equal(vn[g["triangles"]], np.repeat(np.arange(len(n), dtype=np.uint32)[:, None], 3, 1))
+    n, _ = orc.face_normals(g["positions"], g["vertices"], g["triangles"][:, ::-1].copy())
+    assert np.all(n[:, :2] == 0) and np.all(np.abs(n[:, 2] + 1) <= 6e-8)
+    # a tilted triangle: cross((1,0,0), (0,1,1)) = (0,-1,1)
+    pos = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 1], [5, 5, 5]])
+    verts = np.uint32([[0, 0, 7], [1, 0, 7], [2, 0, 7], [3, 0, 9]])
+    n, vn = orc.face_normals(pos, verts, np.uint32([[0, 1, 2]]))
+    assert np.allclose(n[0], [0, -2 ** -0.5, 2 ** -0.5], atol=1e-7, rtol=0) and n[0, 0] == 0
+    assert list(vn) == [0, 0, 0, 9]  # a vertex outside every triangle keeps its index
+    # shared vertices 1 and 2: the second triangle's index wins
+    pos = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]])
+    verts = np.uint32([[i, 0, 0] for i in range(4)])
+    n, vn = orc.face_normals(pos, verts, np.uint32([[0, 1, 2], [2, 1, 3]]))
+    assert list(vn) == [0, 1, 1, 1]
+    assert np.array_equal(n, np.float32([[0, 0, 1], [0, 0, 1]]))
 
 
 def test_face_normals_match_oracle():

@@ -25,7 +25,12 @@ Function -> pins
   orc_hemisphere (AO/BN/thick)  open plane (AO 1), enclosing sphere (AO 0),
                                 wall (blocked fraction from the table), slab
                                 thickness h/cos, BN of an open plane, NaN BN when
-                                fully occluded, invariance under 4 different BVHs
+                                fully occluded, invariance under 4 different BVHs;
+                                hits beyond maxD: counted by BN and returned by
+                                thickness in a leaf the rays enter before maxD,
+                                not in one entered beyond it (mutant oracles with
+                                a t < maxD test in BN, or a clamp in thickness,
+                                fail this pin)
   orc_tangentspace              identity and rotated orthonormal frames
   orc_height / orc_texel_attr   closed forms on a plane
 """
@@ -340,6 +345,67 @@ def test_bent_normal_of_open_plane():
     v = table[:64].astype(np.float64) * np.array([-1.0, 1.0, 1.0])   # tx = -x, ty = +y
     expect = v.sum(0) / np.linalg.norm(v.sum(0))
     assert cnt[0] == 0 and np.allclose(bn[0], expect, atol=1e-5)
+
+
+def beyond_maxd(max_tris, S=64, maxd=1.0, library=None):
+    """A floor at z = 0 (the texel maps onto it), a roof at z = 2 facing down
+    and a basement at z = -2, both 200 wide, with maxD = 1: every roof or
+    basement hit lies beyond maxD.  Returns (tree leaves' z sets, AO, BN,
+    thickness results and counts, sample table)."""
+    m = soup_mesh(quad(0.0, 8.0) + quad(2.0, 200.0, flip=True) + quad(-2.0, 200.0))
+    s, c, t = mapped(m, np.float32([[0.0, 0.0, -0.05]]), np.float32([[0, 0, 1]]), max_tris=max_tris)
+    assert t[0] != MISS and c[0, 0] == np.float32(0.05)
+    leaves = [set(np.round(s["spos"][n[6]:n[7], 2], 3)) for n in s["nodes"] if n[7] > n[6]]
+    out = {}
+    for kind in (orc.KIND_AO, orc.KIND_BN, orc.KIND_THICK):
+        rng = np.random.default_rng(9)
+        table = orc.samples(S, rng.integers(0, 2 ** 32, (16, 2), dtype=np.uint64).astype(np.uint32))
+        out[kind] = orc.hemisphere(kind, S, 1e-3, maxd, table, c, t, s["spos"], s["snrm"], s["nodes"],
+                                   library=library)
+    return leaves, out, table
+
+
+def check_beyond_maxd(library=None):
+    S, maxd = 64, 1.0
+    # frame for the +z normal: tx = -x, ty = +y (ao_step0.comp:47-50); a
+    # sample's ray reaches the 200-wide roof (up) or basement (down) at
+    # height 2 when |2 x / z| and |2 y / z| stay below 100
+    # (a) one leaf holds everything: its box contains the ray origins, so the
+    # reference enters it at t <= 0 < maxD and tests every triangle in it
+    leaves, out, table = beyond_maxd(8, S, maxd, library)
+    assert len(leaves) == 1
+    r = table[:S].astype(np.float64)
+    reach = (np.abs(2 * r[:, 0] / r[:, 2]) < 100) & (np.abs(2 * r[:, 1] / r[:, 2]) < 100)
+    assert np.all(np.abs(np.abs(2 * r[:, :2] / r[:, 2:]) - 100) > 1)  # no sample near the edge
+    ao, ao_cnt = out[orc.KIND_AO]
+    assert ao_cnt[0] == 0 and ao[0] == 1.0            # AO: hits at t >= maxD do not count
+    bn, bn_cnt = out[orc.KIND_BN]
+    assert bn_cnt[0] == int(reach.sum()) > S // 2     # BN: any hit in a reached leaf (bentnormals_step1.comp:170)
+    th, th_cnt = out[orc.KIND_THICK]
+    t = np.where(reach, 2.0 / (r[:, 2] / np.linalg.norm(r, axis=1)), maxd)
+    assert th_cnt[0] == int(reach.sum())              # thickness: closest t, not clamped (thick_step1.comp:158)
+    assert abs(th[0] - t.mean()) <= 1e-5 * t.mean() and th[0] > 1.5
+    # (b) two triangles per leaf: roof and basement sit in leaves of their
+    # own whose boxes the rays enter at t >= 2 > maxD, so the reference never
+    # tests them: no occlusion, thickness maxD
+    leaves, out, _ = beyond_maxd(2, S, maxd, library)
+    assert all(len(z) == 1 for z in leaves) and len(leaves) == 3
+    bn, bn_cnt = out[orc.KIND_BN]
+    assert bn_cnt[0] == 0 and np.all(np.isfinite(bn[0]))
+    th, th_cnt = out[orc.KIND_THICK]
+    assert th_cnt[0] == 0 and th[0] == np.float32(maxd)
+
+
+def test_bn_and_thickness_count_hits_beyond_maxd_only_in_reached_leaves():
+    check_beyond_maxd()
+
+
+@pytest.mark.parametrize("name", ["bn_maxd", "thick_clamp"])
+def test_beyond_maxd_pins_reject_planted_mistakes(name):
+    # the oracle built with a t < maxD test in bent normals, or with
+    # thickness hits clamped to maxD, fails the pins above
+    with pytest.raises(AssertionError):
+        check_beyond_maxd(orc.mutant(name))
 
 
 def test_missed_texels_get_defined_values():
