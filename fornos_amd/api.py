@@ -334,8 +334,9 @@ class Solver(_Handle):
         return p.value
 
     def run_texels(self, texels):
-        """The pass for the listed texels only (increasing global indices);
-        their results equal run_all's (fornos_solver_run_texels)."""
+        """The pass for the listed texels only (distinct global indices, in
+        the order their rays are handed out); their results equal run_all's
+        (fornos_solver_run_texels)."""
         t = np.ascontiguousarray(texels, np.uint32)
         ms = C.c_float()
         check(lib().fornos_solver_run_texels(self._h, ptr(t), len(t), C.byref(ms)))
@@ -398,6 +399,19 @@ class Solver(_Handle):
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(lib().fornos_solver_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    RAY_ORDER_TEXELS, RAY_ORDER_MORTON = 0, 1
+
+    def set_ray_order(self, order):
+        """RAY_ORDER_MORTON (default): texels' rays handed out by the Morton
+        code of their hit positions; RAY_ORDER_TEXELS: the texel set's own
+        order (fornos_solver_set_ray_order).  Results do not depend on it."""
+        check(lib().fornos_solver_set_ray_order(self._h, int(order)))
+
+    def get_ray_order(self):
+        t = C.c_int()
+        check(lib().fornos_solver_get_ray_order(self._h, C.byref(t)))
+        return t.value
 
 
 class HeightSolver(Solver):

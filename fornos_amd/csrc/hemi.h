@@ -79,6 +79,7 @@ struct RayArgs {
     uint32_t refill;                    // refill a wave once this many lanes are idle
 };
 
+
 // Global index of the launch's q-th texel (the list is wave-uniform, so the
 // contiguous case costs one scalar branch).
 __device__ inline uint32_t texel_of(const uint32_t* list, uint32_t t0, uint32_t q)
@@ -586,6 +587,45 @@ __global__ void k_fill(uint32_t* p, size_t n, uint32_t v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Sort key of texel g for the order its rays are handed out in: the Morton
+// code of its hit position, 10 bits per axis over the scene's box (the
+// reference tree's root); texels whose mapping missed (no rays) sort last.
+__device__ inline uint32_t spread3(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_order_keys(DevScene sc, const float4* __restrict__ coords, const uint32_t* __restrict__ tidx,
+                             uint32_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    vals[g] = g;
+    const uint32_t t = tidx[g];
+    if (t == kMiss || sc.n_nodes == 0) {
+        keys[g] = 0xffffffffu;
+        return;
+    }
+    const float4 c = coords[g];
+    const float4 a = sc.spos[t], b = sc.spos[t + 1], d = sc.spos[t + 2];
+    const float p[3] = {a.x * c.y + b.x * c.z + d.x * c.w, a.y * c.y + b.y * c.z + d.y * c.w,
+                        a.z * c.y + b.z * c.z + d.z * c.w};
+    const float4 lo = sc.nodes[0], hi = sc.nodes[1];
+    const float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float u = fminf(fmaxf((p[k] - l[k]) / fmaxf(h[k] - l[k], 1e-30f), 0.0f), 1.0f);  // NaN -> 0
+        key |= spread3((uint32_t)(u * 1023.0f)) << k;
+    }
+    keys[g] = key;
 }
 
 // p[list[i]] = v for the n listed words.

@@ -87,10 +87,12 @@ struct fornos_meshmap {
     uint32_t mark_from = 0;   // texel its first part started at
     float4 *pix = nullptr, *pixt = nullptr, *coords = nullptr;
     uint32_t *tidx = nullptr, *indices = nullptr;
+    uint32_t* order = nullptr;  // texels by the Morton code of their hit positions
+    bool order_valid = false;   // built from the current mapping results
     ~fornos_meshmap()
     {
         if (sc && sc->stream) (void)hipStreamSynchronize(sc->stream);
-        dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx); dev_free(indices);
+        dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx); dev_free(indices); dev_free(order);
     }
 };
 
@@ -111,6 +113,7 @@ struct fornos_solver {
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
+    int ray_order = FORNOS_RAY_ORDER_MORTON;  // order the texels' rays are handed out in
     uint32_t* counter = nullptr;       // k_rays work counter
     // per-launch events of the last timed run: (before k_rays, after k_rays, after k_reduce)
     std::vector<hipEvent_t> events;
@@ -296,7 +299,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
 
 // AO solver `s` and its BN partner over texels [t0, t1) (or list[t0 .. t1))
 // with one traversal (k_rays<kAOBN>), then both reductions.
-fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint32_t* list = nullptr)
+fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint32_t* list)
 {
     fornos_solver* b = s->partner;
     fornos_meshmap* mm = s->mm;
@@ -395,7 +398,55 @@ fornos_status follow(fornos_solver* s, bool was_fused)
     return mark_done(s->partner);
 }
 
-fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool count)
+// The texels of `mm` sorted by the Morton code of their hit positions, built
+// once per mapping result.  Hemisphere passes hand out their rays in this
+// order, so the rays in flight leave from a compact patch of the surface
+// rather than from a strip along a map row; results do not depend on it.
+fornos_status meshmap_order(fornos_meshmap* mm, const uint32_t** out)
+{
+    *out = nullptr;
+    const uint32_t n = mm->n;
+    if (n == 0) return FORNOS_OK;
+    if (!mm->order_valid) {
+        hipStream_t st = mm->sc->stream;
+        if (!mm->order)
+            if (fornos_status e = dev_alloc(&mm->order, n)) return e;
+        uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr;
+        void* tmp = nullptr;
+        size_t tmp_bytes = 0;
+        fornos_status e = dev_alloc(&keys, n);
+        if (!e) e = dev_alloc(&keys2, n);
+        if (!e) e = dev_alloc(&vals, n);
+        if (!e && hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, vals, mm->order, (int)n, 0, 32, st) !=
+                      hipSuccess)
+            e = set_error(FORNOS_HIP_ERROR, "texel order: radix sort size query failed");
+        if (!e) e = dev_alloc((uint8_t**)&tmp, std::max<size_t>(tmp_bytes, 1));
+        if (!e) {
+            hipLaunchKernelGGL(k_order_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, mm->sc->d, mm->coords, mm->tidx,
+                               n, keys, vals);
+            if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, vals, mm->order, (int)n, 0, 32, st) !=
+                hipSuccess)
+                e = set_error(FORNOS_HIP_ERROR, "texel order: radix sort failed");
+        }
+        if (hipStreamSynchronize(st) != hipSuccess && !e) e = set_error(FORNOS_HIP_ERROR, "texel order: stream failed");
+        dev_free(keys); dev_free(keys2); dev_free(vals); dev_free((uint8_t*)tmp);
+        if (e) return e;
+        mm->order_valid = true;
+    }
+    *out = mm->order;
+    return FORNOS_OK;
+}
+
+// The order a solver's passes hand out their texels in (null: the texel
+// set's own order).
+fornos_status pass_order(fornos_solver* s, const uint32_t** list)
+{
+    *list = nullptr;
+    if (!s->hemi() || s->ray_order != FORNOS_RAY_ORDER_MORTON) return FORNOS_OK;
+    return meshmap_order(s->mm, list);
+}
+
+fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool count, const uint32_t* list = nullptr)
 {
     fornos_meshmap* mm = s->mm;
     fornos_scene* sc = mm->sc;
@@ -413,9 +464,9 @@ fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool cou
         hipLaunchKernelGGL(k_attr<true>, dim3(blocks), dim3(kBlock), 0, sc->stream, mm->coords, mm->tidx,
                            sc->snrm, mm->pixt, t0, t1, s->p.tangent_space, s->out);
         break;
-    case FORNOS_SOLVER_AO: return launch_hemi<kAO>(s, t0, t1, count, own_bufs(s));
-    case FORNOS_SOLVER_BENT_NORMALS: return launch_hemi<kBN>(s, t0, t1, count, own_bufs(s));
-    case FORNOS_SOLVER_THICKNESS: return launch_hemi<kThick>(s, t0, t1, count, own_bufs(s));
+    case FORNOS_SOLVER_AO: return launch_hemi<kAO>(s, t0, t1, count, own_bufs(s), list);
+    case FORNOS_SOLVER_BENT_NORMALS: return launch_hemi<kBN>(s, t0, t1, count, own_bufs(s), list);
+    case FORNOS_SOLVER_THICKNESS: return launch_hemi<kThick>(s, t0, t1, count, own_bufs(s), list);
     }
     HIP_TRY(hipGetLastError());
     return FORNOS_OK;
@@ -790,6 +841,7 @@ fornos_status fornos_meshmap_read_texels(const fornos_meshmap* mm, float* p, flo
 static fornos_status meshmap_launch(fornos_meshmap* mm, uint32_t t0, uint32_t t1)
 {
     if (t1 <= t0) return FORNOS_OK;
+    mm->order_valid = false;
     const uint32_t blocks = div_up(t1 - t0, kBlock);
     if (mm->cull)
         hipLaunchKernelGGL(k_meshmap<true>, dim3(blocks), dim3(kBlock), 0, mm->sc->stream, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
@@ -1013,7 +1065,10 @@ fornos_status fornos_solver_run_step(fornos_solver* s, int* done)
             const uint64_t work = std::min<uint64_t>(s->work_total - s->offset, per);
             const uint32_t t0 = (uint32_t)(s->offset / S), t1 = (uint32_t)std::min<uint64_t>((s->offset + work) / S, n);
             const bool fz = fused(s);
-            if (fornos_status st = fz ? launch_pair(s, std::min(t0, n), t1) : solver_launch(s, std::min(t0, n), t1, false))
+            const uint32_t* list = nullptr;
+            if (fornos_status st = pass_order(s, &list)) return st;
+            if (fornos_status st = fz ? launch_pair(s, std::min(t0, n), t1, list)
+                                      : solver_launch(s, std::min(t0, n), t1, false, list))
                 return st;
             s->offset += work;
             if (fornos_status st = follow(s, fz)) return st;
@@ -1039,9 +1094,11 @@ fornos_status fornos_solver_run_all(fornos_solver* s, float* ms)
     if (fornos_status st = tm.start()) return st;
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     s->events.clear();
+    const uint32_t* list = nullptr;
+    if (fornos_status st = pass_order(s, &list)) return st;
     s->timed = s->hemi();
     const bool fz = fused(s);
-    const fornos_status st = fz ? launch_pair(s, t0, n) : solver_launch(s, t0, n, false);
+    const fornos_status st = fz ? launch_pair(s, t0, n, list) : solver_launch(s, t0, n, false, list);
     s->timed = false;
     if (st) return st;
     s->offset = s->work_total;
@@ -1065,10 +1122,17 @@ fornos_status fornos_solver_run_texels(fornos_solver* s, const uint32_t* texels,
     if (!s->hemi())
         return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: not a hemisphere solver (AO, bent normals, thickness)");
     const uint32_t count = s->mm->n;
-    for (uint32_t i = 0; i < n; ++i)
-        if (texels[i] >= count || (i && texels[i] <= texels[i - 1]))
-            return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: texel indices must be increasing and below the "
-                                                 "texel count");
+    {
+        // distinct and in range: a texel listed twice would count its rays twice
+        std::vector<uint64_t> seen(((size_t)count + 63) / 64, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t t = texels[i];
+            if (t >= count || (seen[t >> 6] >> (t & 63)) & 1)
+                return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: texel indices must be distinct and below "
+                                                     "the texel count");
+            seen[t >> 6] |= 1ull << (t & 63);
+        }
+    }
     if (s->owner) return set_error(FORNOS_INVALID_ARG, "fornos_solver_run_texels: a paired bent-normals solver runs through its AO solver");
     hipStream_t st = s->mm->sc->stream;
     if (n > s->dlist_cap) {
@@ -1155,10 +1219,12 @@ fornos_status fornos_solver_submit(fornos_solver* s, uint32_t parts)
     const uint32_t t0 = (uint32_t)std::min<uint64_t>(s->offset / S, n);
     s->mark_from = t0;
     const bool fz = fused(s);
+    const uint32_t* list = nullptr;
+    if (fornos_status st = pass_order(s, &list)) return st;
     const uint32_t per = std::max<uint32_t>(1, div_up(n - t0, parts));
     for (uint32_t a = t0; a < n; a += per) {
         const uint32_t b = std::min(n, a + per);
-        if (fornos_status st = fz ? launch_pair(s, a, b) : solver_launch(s, a, b, false)) return st;
+        if (fornos_status st = fz ? launch_pair(s, a, b, list) : solver_launch(s, a, b, false, list)) return st;
         if (fornos_status st = s->marks.add(s->mm->sc->stream, b)) return st;
     }
     s->offset = s->work_total;
@@ -1238,7 +1304,9 @@ fornos_status fornos_solver_count_work(fornos_solver* s, uint64_t* nodes, uint64
     if (s->mm->n)
         hipLaunchKernelGGL(k_fill, dim3(div_up(s->mm->n, kBlock)), dim3(kBlock), 0, s->mm->sc->stream, s->fbflags,
                            (size_t)s->mm->n, 0u);
-    if (fornos_status st = solver_launch(s, 0, s->mm->n, true)) return st;
+    const uint32_t* list = nullptr;
+    if (fornos_status st = pass_order(s, &list)) return st;
+    if (fornos_status st = solver_launch(s, 0, s->mm->n, true, list)) return st;
     unsigned long long h[kStats] = {};
     HIP_TRY(hipStreamSynchronize(s->mm->sc->stream));
     HIP_TRY(hipMemcpy(h, s->stats, sizeof(h), hipMemcpyDeviceToHost));
@@ -1277,6 +1345,21 @@ fornos_status fornos_solver_set_launch(fornos_solver* s, uint32_t blocks_per_cu,
     s->blocks_per_cu = blocks_per_cu;
     s->refill = refill;
     s->stack = stack;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_set_ray_order(fornos_solver* s, int order)
+{
+    if (!s || (order != FORNOS_RAY_ORDER_TEXELS && order != FORNOS_RAY_ORDER_MORTON))
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_set_ray_order: FORNOS_RAY_ORDER_TEXELS or FORNOS_RAY_ORDER_MORTON");
+    s->ray_order = order;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_get_ray_order(const fornos_solver* s, int* order)
+{
+    if (!s || !order) return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_ray_order: null argument");
+    *order = s->ray_order;
     return FORNOS_OK;
 }
 
@@ -1324,8 +1407,13 @@ fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, R
     hipStream_t st = b->stream;
     const uint32_t nj = (uint32_t)jobs.size();
     hipLaunchKernelGGL(k_fill, dim3(div_up(nj, kBlock)), dim3(kBlock), 0, st, b->counters, (size_t)nj, 0u);
-    for (const RayArgs& J : jobs)
-        if (J.ntex) hipLaunchKernelGGL(k_fill, dim3(div_up(J.ntex, kBlock)), dim3(kBlock), 0, st, J.cnt, (size_t)J.ntex, 0u);
+    for (const RayArgs& J : jobs) {
+        if (!J.ntex) continue;
+        if (J.list)
+            hipLaunchKernelGGL(k_fill_list, dim3(div_up(J.ntex, kBlock)), dim3(kBlock), 0, st, J.cnt, J.list, J.ntex, 0u);
+        else
+            hipLaunchKernelGGL(k_fill, dim3(div_up(J.ntex, kBlock)), dim3(kBlock), 0, st, J.cnt, (size_t)J.ntex, 0u);
+    }
     uint64_t rays = 0;
     for (const RayArgs& J : jobs) rays += (uint64_t)J.ntex * J.S;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)b->cus * b->blocks_per_cu, div_up(rays, kBlock));
@@ -1340,6 +1428,7 @@ fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, R
         R.S = jobs[j].S;
         R.t0 = 0;
         R.ntex = jobs[j].ntex;
+        R.list = jobs[j].list;
         if (R.ntex) hipLaunchKernelGGL(k_reduce<kAO>, dim3(div_up(R.ntex, kBlock)), dim3(kBlock), 0, st, R);
     }
     HIP_TRY(hipGetLastError());
@@ -1376,7 +1465,7 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
     if (fornos_status st = dev_alloc(&b->counters, n)) return st;
     if (fornos_status st = dev_alloc(&b->djobs, n)) return st;
     for (uint32_t j = 0; j < n; ++j) {
-        const fornos_solver* s = b->solvers[j];
+        fornos_solver* s = b->solvers[j];
         RayArgs A{};
         A.sc = s->mm->sc->d;
         A.coords = s->mm->coords;
@@ -1393,6 +1482,9 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         A.mind = s->p.min_distance;
         A.maxd = s->p.max_distance;
         A.refill = b->refill;
+        const uint32_t* list = nullptr;
+        if (fornos_status st = pass_order(s, &list)) return st;
+        A.list = list;
         b->jobs.push_back(A);
     }
     HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * n, hipMemcpyHostToDevice));

@@ -48,8 +48,12 @@ def test_run_texels_equals_run_all(field, kind):
     if kind == "paired":
         assert same_bits(bn_part.getResults()[texels], bn_full.getResults()[texels])
         assert np.array_equal(bn_part.counts()[texels], bn_full.counts()[texels])
+    # any order: the same results
+    again = field.solver(k)
+    again.run_texels(np.random.default_rng(6).permutation(texels))
+    assert same_bits(again.getResults()[texels], full.getResults()[texels])
     with pytest.raises(fa.FornosError):
-        part.run_texels(texels[::-1])  # not increasing
+        part.run_texels(np.concatenate([texels, texels[:1]]))  # listed twice
     with pytest.raises(fa.FornosError):
         part.run_texels([n])  # beyond the texel set
 
