@@ -400,12 +400,13 @@ class Solver(_Handle):
         check(lib().fornos_solver_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
-    RAY_ORDER_TEXELS, RAY_ORDER_MORTON = 0, 1
+    RAY_ORDER_TEXELS, RAY_ORDER_MORTON, RAY_ORDER_AUTO = 0, 1, 2
 
     def set_ray_order(self, order):
-        """RAY_ORDER_MORTON (default): texels' rays handed out by the Morton
-        code of their hit positions; RAY_ORDER_TEXELS: the texel set's own
-        order (fornos_solver_set_ray_order).  Results do not depend on it."""
+        """RAY_ORDER_MORTON: texels' rays handed out by the Morton code of
+        their hit positions; RAY_ORDER_TEXELS: the texel set's own order;
+        RAY_ORDER_AUTO (default): Morton for rays that cross the scene
+        (fornos_solver_set_ray_order).  Results do not depend on it."""
         check(lib().fornos_solver_set_ray_order(self._h, int(order)))
 
     def get_ray_order(self):

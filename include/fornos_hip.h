@@ -310,17 +310,22 @@ fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu,
 fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
                                        uint32_t *stack);
 /* Order in which a hemisphere solver's passes hand out their texels' rays.
- * FORNOS_RAY_ORDER_MORTON (default): texels sorted by the Morton code of
- * their hit positions (built on the device once per mapping result), so the
- * rays in flight leave from a compact patch of the surface; on the 8.4M-
- * triangle knot this cuts AO-512 and thickness-256 by 9 and 18 %.
+ * FORNOS_RAY_ORDER_MORTON: texels sorted by the Morton code of their hit
+ * positions (built on the device once per mapping result), so the rays in
+ * flight leave from a compact patch of the surface; on the 8.4M-triangle
+ * knot (maxD 10) this cuts AO-512 and thickness-256 by 10 and 20 %.
  * FORNOS_RAY_ORDER_TEXELS: the texel set's own order (MapUV rows), as the
- * reference dispatches them (Shaders/ao_step1.comp:140-157).  Each texel's
- * rays keep their sample permutation (texel % n_perm) and its results their
- * place, so results do not depend on the order; a run_step covers the next
- * 131072 rays of the order.  No reference counterpart. */
+ * reference dispatches them (Shaders/ao_step1.comp:140-157).
+ * FORNOS_RAY_ORDER_AUTO (default): Morton when maxDistance is at least a
+ * quarter of the scene box's diagonal (rays that cross the scene), else the
+ * texel order (short rays stay near a map row's texels, which already lie
+ * together: heightfield2048's paired AO + BN pass is 3 % faster so).
+ * Each texel's rays keep their sample permutation (texel % n_perm) and its
+ * results their place, so results do not depend on the order; a run_step
+ * covers the next 131072 rays of the order.  No reference counterpart. */
 #define FORNOS_RAY_ORDER_TEXELS 0
 #define FORNOS_RAY_ORDER_MORTON 1
+#define FORNOS_RAY_ORDER_AUTO 2
 fornos_status fornos_solver_set_ray_order(fornos_solver *s, int order);
 fornos_status fornos_solver_get_ray_order(const fornos_solver *s, int *order);
 void fornos_solver_destroy(fornos_solver *s);

@@ -167,13 +167,13 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     cls = {"ao": fa.AmbientOcclusionSolver, "bentnormals": fa.BentNormalsSolver,
            "thickness": fa.ThicknessSolver}[kind]
     s = cls(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
-    assert s.get_launch() == (8, 48, 16) and s.get_ray_order() == s.RAY_ORDER_MORTON
+    assert s.get_launch() == (8, 48, 16) and s.get_ray_order() == s.RAY_ORDER_AUTO
     s.run_all()
     ref = s.getResults().view(np.uint32).copy()
     live = int(np.sum(field.tidx != MISS))
     for k, shape in enumerate([(8, 16, 24), (8, 16, 32), (1, 1, 16), (3, 64, 24), (16, 7, 32)]):
         s.set_launch(*shape)
-        s.set_ray_order(s.RAY_ORDER_TEXELS if k % 2 == 0 else s.RAY_ORDER_MORTON)
+        s.set_ray_order((s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON, s.RAY_ORDER_AUTO)[k % 3])
         assert s.get_launch() == shape
         s.reset()
         s.run_all()
@@ -186,7 +186,7 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     with pytest.raises(fa.FornosError):
         s.set_launch(8, 16, 20)
     with pytest.raises(fa.FornosError):
-        s.set_ray_order(2)
+        s.set_ray_order(3)
 
 
 @pytest.mark.parametrize("maxd", [0.05, 0.3])
