@@ -35,16 +35,17 @@ struct Layer {
     uint64_t f_off;         // in the flipped-weight buffer
 };
 
+// Tiles of 8 x 32 output pixels, for 64 output channels (or 32 when cout is
+// not a multiple of 64) per block.
 void launch_conv(const ConvArgs& A, hipStream_t st)
 {
-    const uint64_t P = (uint64_t)A.n * A.h * A.w;
-    if (A.cout % 64 == 0) {
-        const dim3 grid(div_up(P, 4 * 32), A.cout / 64);
-        hipLaunchKernelGGL((k_conv3x3<2, 4>), grid, dim3(kThreads), 0, st, A);
-    } else {
-        const dim3 grid(div_up(P, 4 * 64), A.cout / 32);
-        hipLaunchKernelGGL((k_conv3x3<4, 2>), grid, dim3(kThreads), 0, st, A);
-    }
+    constexpr int MT = 4, TH = 2 * MT, TW = 32;
+    const uint32_t tx = div_up(A.w, TW), ty = div_up(A.h, TH);
+    const uint32_t tiles = A.n * tx * ty;
+    if (A.cout % 64 == 0)
+        hipLaunchKernelGGL((k_conv3x3<MT, 4>), dim3(tiles, A.cout / 64), dim3(kThreads), 0, st, A, tx, ty);
+    else
+        hipLaunchKernelGGL((k_conv3x3<MT, 2>), dim3(tiles, A.cout / 32), dim3(kThreads), 0, st, A, tx, ty);
 }
 
 void launch_wgrad(WgradArgs A, hipStream_t st)

@@ -6,7 +6,7 @@
 //
 // Tensors are NHWC: pixel-major, channels contiguous, bf16 stored as uint16_t
 // bit patterns.  A 16-pixel x 32-channel MFMA operand is then one 16-byte
-// load per lane: lane l takes pixel (l & 15) and channels 8*(l >> 4) .. +7.
+// read per lane: lane l takes pixel (l & 15) and channels 8*(l >> 4) .. +7.
 // Every convolution input has a multiple of 32 channels.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -64,80 +64,162 @@ struct ConvArgs {
     int up_a, relu, out_f32, acc2;
 };
 
+// LDS-tiled implicit GEMM.  A block computes a TH x 32 tile of output
+// pixels (TH = 2 * MT rows, MT / 2 rows per wave) for NB = 16 * NT output
+// channels.  For each 32-channel slice of the input it stages the tile plus
+// its one-pixel halo ((TH + 2) x 34 pixels, 64 bytes each) and the slice's
+// weights for the 9 taps ([tap][cout][32 channels]) in LDS once; the 9 taps
+// then read their shifted 16-pixel operands from LDS.  A slice's global loads
+// are all issued before any is stored to LDS (one latency per slice, not one
+// per load), and the next slice's loads are in flight while the current
+// slice's MFMAs run.  Pixels outside the image read as zero (the zero
+// padding) and are not written.
+//
+// LDS layout: 16-byte word q (channels 8q .. 8q + 7) of row p (a pixel, or a
+// tap's output channel) sits at word 4p + q.
+__device__ inline int lds_word(int p, int q) { return 4 * p + q; }
+
 template <int MT, int NT>
-__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A)
+__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles_x, uint32_t tiles_y)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int TW = 32, TH = 2 * MT, IW = TW + 2, IH = TH + 2, NB = 16 * NT;
+    // [pixel][4 x 8 channels] and [tap][cout][4 x 8 channels]; after the last
+    // slice, the block's bf16 outputs [pixel][NB channels] (forward pass)
+    constexpr int NLDS = IH * IW * 4 + 9 * NB * 4;
+    static_assert(TH * TW * NB * 2 <= NLDS * 16, "output tile fits the staging LDS");
+    __shared__ uint4 s_lds[NLDS];
+    uint4* const s_in = s_lds;
+    uint4* const s_wt = s_lds + IH * IW * 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, kq = lane >> 4;
-    const uint32_t hw = A.h * A.w, P = A.n * hw;
-    const uint32_t pbase = (blockIdx.x * 4 + wave) * (MT * 16);
-    const uint32_t cbase = blockIdx.y * (NT * 16);
+    uint32_t bid = blockIdx.x;
+    const uint32_t tx = bid % tiles_x;
+    bid /= tiles_x;
+    const uint32_t ty = bid % tiles_y, img = bid / tiles_y;
+    const int H = (int)A.h, W = (int)A.w;
+    const int y0 = (int)ty * TH, x0 = (int)tx * TW;
+    const uint32_t cbase = blockIdx.y * NB;
     const uint32_t cin = A.c1 + A.c2;
     const uint32_t hh = A.h >> 1, wh = A.w >> 1;
-    int py[MT], px[MT], pn[MT];
-    bool pv[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        const uint32_t p = pbase + m * 16 + r;
-        pv[m] = p < P;
-        const uint32_t q = pv[m] ? p : 0;
-        pn[m] = q / hw;
-        const uint32_t rem = q - pn[m] * hw;
-        py[m] = rem / A.w;
-        px[m] = rem - py[m] * A.w;
-    }
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0, 0, 0, 0};
 
-    for (int t = 0; t < 9; ++t) {
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
-        size_t ia[MT], ib[MT];
-        bool sv[MT];
+    constexpr int NIN = IH * IW * 4, NWT = 9 * NB * 4;
+    constexpr int KIN = (NIN + kThreads - 1) / kThreads, KWT = (NWT + kThreads - 1) / kThreads;
+    uint4 rin[KIN], rwt[KWT];
+    auto fetch = [&](uint32_t c0) {
+        const bool from_a = c0 < A.c1;
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int y = py[m] + dy, x = px[m] + dx;
-            sv[m] = pv[m] && y >= 0 && y < (int)A.h && x >= 0 && x < (int)A.w;
-            const int yc = sv[m] ? y : 0, xc = sv[m] ? x : 0;
-            ia[m] = A.up_a ? (((size_t)pn[m] * hh + (yc >> 1)) * wh + (xc >> 1)) * A.c1
-                           : (((size_t)pn[m] * A.h + yc) * A.w + xc) * A.c1;
-            ib[m] = (((size_t)pn[m] * A.h + yc) * A.w + xc) * A.c2;
+        for (int k = 0; k < KIN; ++k) {
+            const int i = tid + k * kThreads;
+            const int q = i & 3, pix = i >> 2;
+            const int iy = pix / IW, ix = pix - iy * IW;
+            const int y = y0 + iy - 1, x = x0 + ix - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < NIN && y >= 0 && y < H && x >= 0 && x < W) {
+                const uint16_t* src;
+                if (!from_a)
+                    src = A.b + (((size_t)img * H + y) * W + x) * A.c2 + (c0 - A.c1) + q * 8;
+                else if (A.up_a)
+                    src = A.a + (((size_t)img * hh + (y >> 1)) * wh + (x >> 1)) * A.c1 + c0 + q * 8;
+                else
+                    src = A.a + (((size_t)img * H + y) * W + x) * A.c1 + c0 + q * 8;
+                v = *reinterpret_cast<const uint4*>(src);
+            }
+            rin[k] = v;
         }
-        const uint16_t* wt = A.wts + (size_t)t * cin + kq * 8;
-        for (uint32_t c0 = 0; c0 < cin; c0 += 32) {
-            bf16x8 af[MT], bw[NT];
-            const bool from_a = c0 < A.c1;
+#pragma unroll
+        for (int k = 0; k < KWT; ++k) {
+            const int i = tid + k * kThreads;
+            const int q = i & 3, rest = i >> 2;
+            const int co = rest % NB, t = rest / NB;
+            rwt[k] = i < NWT ? *reinterpret_cast<const uint4*>(A.wts + ((size_t)(cbase + co) * 9 + t) * cin + c0 + q * 8)
+                             : make_uint4(0, 0, 0, 0);
+        }
+    };
+    fetch(0);
+    for (uint32_t c0 = 0; c0 < cin; c0 += 32) {
+#pragma unroll
+        for (int k = 0; k < KIN; ++k) {
+            const int i = tid + k * kThreads;
+            if (i < NIN) s_in[lds_word(i >> 2, i & 3)] = rin[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KWT; ++k) {
+            const int i = tid + k * kThreads;
+            if (i < NWT) s_wt[lds_word(i >> 2, i & 3)] = rwt[k];
+        }
+        __syncthreads();
+        if (c0 + 32 < cin) fetch(c0 + 32);
+#pragma unroll 3
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t - 3 * (t / 3);  // offsets in the haloed tile
+            bf16x8 bw[NT];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) bw[j] = __builtin_bit_cast(bf16x8, s_wt[lds_word(t * NB + j * 16 + r, kq)]);
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
-                const uint16_t* src = from_a ? A.a + ia[m] + c0 + kq * 8 : A.b + ib[m] + (c0 - A.c1) + kq * 8;
-                af[m] = sv[m] ? load8(src) : zero8();
-            }
-#pragma unroll
-            for (int j = 0; j < NT; ++j) bw[j] = load8(wt + (size_t)(cbase + j * 16 + r) * 9 * cin + c0);
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
+                const int row = wave * (MT / 2) + (m >> 1), col = (m & 1) * 16 + r;
+                const bf16x8 af = __builtin_bit_cast(bf16x8, s_in[lds_word((row + dy) * IW + col + dx, kq)]);
 #pragma unroll
                 for (int j = 0; j < NT; ++j)
-                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bw[j], acc[m][j], 0, 0, 0);
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw[j], acc[m][j], 0, 0, 0);
+            }
         }
+        __syncthreads();  // the slice's LDS reads are done before the next is staged
     }
-    // D[m][n]: lane holds rows 4*kq .. 4*kq + 3 of column r
+    // D[m][n]: lane holds pixels 4*kq .. 4*kq + 3 of its fragment, output channel r
+    if (!A.out_f32 && A.split == A.cout && !A.mask1) {
+        // forward pass: bf16 outputs through LDS, then 16-byte stores of 8
+        // consecutive channels of a pixel (the LDS reads of the last slice are
+        // done: the loop ends with a barrier)
+        uint16_t* s_out = reinterpret_cast<uint16_t*>(s_lds);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int cl = j * 16 + r;
+            const float bias = A.bias ? A.bias[cbase + cl] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const int row = wave * (MT / 2) + (m >> 1);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = acc[m][j][e] + bias;
+                    if (A.relu) v = v > 0.0f ? v : 0.0f;
+                    s_out[(row * TW + (m & 1) * 16 + kq * 4 + e) * NB + cl] = f2bf(v);
+                }
+            }
+        }
+        __syncthreads();
+        constexpr int CH = NB / 8;  // 16-byte chunks per pixel
+        uint16_t* out = static_cast<uint16_t*>(A.out);
+        for (int i = tid; i < TH * TW * CH; i += kThreads) {
+            const int pix = i / CH, c = i - pix * CH;
+            const int y = y0 + pix / TW, x = x0 + pix % TW;
+            if (y >= H || x >= W) continue;
+            *reinterpret_cast<uint4*>(out + (((size_t)img * H + y) * W + x) * A.cout + cbase + 8 * c) =
+                *reinterpret_cast<const uint4*>(s_out + pix * NB + 8 * c);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         const uint32_t co = cbase + j * 16 + r;
         const float bias = A.bias ? A.bias[co] : 0.0f;
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MT; ++m) {
+            const int y = y0 + wave * (MT / 2) + (m >> 1);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const uint32_t p = pbase + m * 16 + kq * 4 + e;
-                if (p >= P) continue;
+                const int x = x0 + (m & 1) * 16 + kq * 4 + e;
+                if (y >= H || x >= W) continue;
+                const size_t p = ((size_t)img * H + y) * W + x;
                 float v = acc[m][j][e] + bias;
                 if (A.relu) v = v > 0.0f ? v : 0.0f;
                 if (co < A.split) {
-                    const size_t o = (size_t)p * A.split + co;
+                    const size_t o = p * A.split + co;
                     if (A.mask1 && !bf_pos(A.mask1[o])) v = 0.0f;
                     if (A.out_f32)
                         static_cast<float*>(A.out)[o] = v;
@@ -145,13 +227,14 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A)
                         static_cast<uint16_t*>(A.out)[o] = f2bf(v);
                 } else {
                     const uint32_t c2o = A.cout - A.split;
-                    const size_t o = (size_t)p * c2o + (co - A.split);
+                    const size_t o = p * c2o + (co - A.split);
                     if (A.mask2 && !bf_pos(A.mask2[o])) v = 0.0f;
                     uint16_t* o2 = static_cast<uint16_t*>(A.out2);
                     if (A.acc2) v += bf2f(o2[o]);
                     o2[o] = f2bf(v);
                 }
             }
+        }
     }
 }
 
