@@ -433,11 +433,16 @@ __global__ void k_up_back(const uint16_t* gbig, const uint16_t* act, uint16_t* g
 // Network input: fp32 [pixels][cin] -> bf16 [pixels][32], channels past cin zero.
 __global__ void k_pack_input(const float* in, uint16_t* out, size_t pixels, uint32_t cin)
 {
+    // one thread per 8 channels of a pixel: one 16-byte store
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pixels * 32) return;
-    const uint32_t c = (uint32_t)(i & 31);
-    const size_t p = i >> 5;
-    out[i] = c < cin ? f2bf(in[p * cin + c]) : (uint16_t)0;
+    if (i >= pixels * 4) return;
+    const uint32_t c0 = (uint32_t)(i & 3) * 8;
+    const size_t p = i >> 2;
+    uint4 o;
+    uint16_t* o8 = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o8[j] = c0 + j < cin ? f2bf(in[p * cin + c0 + j]) : (uint16_t)0;
+    *reinterpret_cast<uint4*>(out + p * 32 + c0) = o;
 }
 
 // 1x1 head: out[p][o] = res[p][o] + hb[o] + sum_c hw[o][c] * y[p][c]
