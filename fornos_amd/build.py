@@ -60,5 +60,21 @@ def build(verbose=False):
     return LIB
 
 
+SHIM_SRC = os.path.join(ROOT, "tests", "cxx", "shim_bake.cpp")
+SHIM_BIN = os.path.join(ROOT, "tests", "cxx", "shim_bake")
+
+
+def build_shim_check():
+    """Compiles the C++ host program of the reference-signature classes
+    (fornos_amd/csrc/fornos.hpp) that the GPU tests run against the C ABI:
+    only the public headers and the built library, as a C++ caller would."""
+    tmp = SHIM_BIN + ".tmp"
+    _run([CLANGXX, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), SHIM_SRC, "-o", tmp,
+          "-L", HERE, "-lfornos_hip", "-Wl,-rpath,$ORIGIN/../../fornos_amd"])
+    os.replace(tmp, SHIM_BIN)
+    return SHIM_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_shim_check())
