@@ -238,8 +238,12 @@ struct PlyReader {
 
 fornos_status load_ply(const char* path, fornos_mesh& m)
 {
-    std::ifstream in(path, std::ios::binary);
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
     if (!in) return set_error(FORNOS_INVALID_ARG, std::string("cannot open ") + path);
+    // every list entry takes at least one byte of the file: a list count
+    // above the file's size is corrupt (and must not size an allocation)
+    const double file_bytes = (double)in.tellg();
+    in.seekg(0);
     auto bad = [&](const std::string& what) { return set_error(FORNOS_INVALID_ARG, std::string(path) + ": " + what); };
     std::string l;
     if (!std::getline(in, l) || l.compare(0, 3, "ply") != 0) return bad("not a PLY file");
@@ -295,7 +299,9 @@ fornos_status load_ply(const char* path, fornos_mesh& m)
             for (const PlyProp& p : e.props) {
                 if (!p.count_type.empty()) {
                     double cnt;
-                    if (!r.value(p.count_type, cnt) || cnt < 0) return bad("truncated list");
+                    if (!r.value(p.count_type, cnt)) return bad("truncated list");
+                    if (!(cnt >= 0 && cnt <= file_bytes) || cnt != std::floor(cnt))
+                        return bad("bad list count");
                     std::vector<uint32_t> idx((size_t)cnt);
                     for (size_t k = 0; k < idx.size(); ++k) {
                         double v;
@@ -388,15 +394,21 @@ fornos_status fornos_mesh_load(const char* path, fornos_mesh** out)
 {
     if (!path || !out) return set_error(FORNOS_INVALID_ARG, "fornos_mesh_load: null argument");
     *out = nullptr;
-    std::unique_ptr<fornos_mesh> m(new fornos_mesh());
-    const std::string p(path);
-    fornos_status st;
-    if (ends_with(p, ".obj")) st = load_obj(path, *m);
-    else if (ends_with(p, ".ply")) st = load_ply(path, *m);
-    else return set_error(FORNOS_INVALID_ARG, std::string("unsupported mesh file: ") + path);
-    if (st) return st;
-    *out = m.release();
-    return FORNOS_OK;
+    try {
+        std::unique_ptr<fornos_mesh> m(new fornos_mesh());
+        const std::string p(path);
+        fornos_status st;
+        if (ends_with(p, ".obj")) st = load_obj(path, *m);
+        else if (ends_with(p, ".ply")) st = load_ply(path, *m);
+        else return set_error(FORNOS_INVALID_ARG, std::string("unsupported mesh file: ") + path);
+        if (st) return st;
+        *out = m.release();
+        return FORNOS_OK;
+    } catch (const std::bad_alloc&) {
+        return set_error(FORNOS_OOM, std::string("fornos_mesh_load: out of memory reading ") + path);
+    } catch (const std::exception& e) {
+        return set_error(FORNOS_INVALID_ARG, std::string("fornos_mesh_load: ") + path + ": " + e.what());
+    }
 }
 
 fornos_status fornos_mesh_compute_face_normals(fornos_mesh* m)

@@ -210,3 +210,30 @@ def test_unknown_extension_is_an_error(tmp_path):
     p.write_text("solid")
     with pytest.raises(fa.FornosError):
         fa.Mesh.load(p)
+
+
+@pytest.mark.parametrize("count", ["4294967295", "1e300", "nan", "2.5", "-1"])
+def test_ply_corrupt_list_count_is_an_error(tmp_path, count):
+    # a face list whose count is huge, not finite, fractional or negative:
+    # rejected with a status before any allocation sized by it
+    p = tmp_path / "m.ply"
+    p.write_text("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                 "property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n"
+                 "0 0 0\n1 0 0\n0 1 0\n%s 0 1 2\n" % count)
+    with pytest.raises(fa.FornosError) as e:
+        fa.Mesh.load(p)
+    assert e.value.status == _abi.INVALID_ARG
+
+
+def test_ply_binary_oversized_count_is_an_error(tmp_path):
+    # binary little-endian: a uint count of 0xFFFFFFFF in a short file
+    head = ("ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+            "property float z\nelement face 1\nproperty list uint int vertex_indices\nend_header\n").encode()
+    body = struct.pack("<9f", 0, 0, 0, 1, 0, 0, 0, 1, 0) + struct.pack("<I", 0xFFFFFFFF) + struct.pack("<3i", 0, 1, 2)
+    p = tmp_path / "m.ply"
+    p.write_bytes(head + body)
+    with pytest.raises(fa.FornosError):
+        fa.Mesh.load(p)
+    p.write_bytes(head + body[:40])  # truncated inside the face element
+    with pytest.raises(fa.FornosError):
+        fa.Mesh.load(p)
