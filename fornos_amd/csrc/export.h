@@ -143,17 +143,21 @@ __global__ void k_valid_rgb(const uint32_t* indices, uint32_t n, uint32_t w, uin
     valid[idx] = (rgb[p] != 0 || rgb[p + 1] != 0 || rgb[p + 2] != 0) ? 1 : 0;
 }
 
-// dilateRGB (Src/image.cpp:127-183), one thread per pixel: an invalid pixel
-// takes the colour of the first centre pos + offset * dist (dist = 1 ..
-// max_dist - 1, the eight offsets in the reference's order) that lies inside
-// the border and whose eight neighbours are all valid.  Colours are read
-// from the image before dilation.
-__global__ void k_dilate(const uint8_t* src, uint8_t* dst, const uint8_t* valid, uint32_t w, uint32_t h,
-                         uint32_t max_dist)
+// dilateRGB (Src/image.cpp:127-183) as it runs on one thread: the pixels in
+// the map's row-major order (y, then x), each invalid one filled in place.
+// Pass 1, one thread per pixel: an invalid pixel's centre, the first
+// pos + offset * dist (dist = 1 .. max_dist - 1, the eight offsets in the
+// reference's order) strictly inside the border whose eight neighbours are all
+// valid, as a map index (kNoCentre: none).
+constexpr uint32_t kNoCentre = 0xffffffffu;
+
+__global__ void k_dilate_centre(const uint8_t* valid, uint32_t* centre, uint32_t w, uint32_t h, uint32_t max_dist)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w || y >= h) return;
-    if (valid[(size_t)y * w + x]) return;
+    const size_t p = (size_t)y * w + x;
+    centre[p] = kNoCentre;
+    if (valid[p]) return;
     const int ox[8] = {1, -1, 0, 0, 1, 1, -1, -1}, oy[8] = {0, 0, 1, -1, 1, -1, 1, -1};
     const int W = (int)w, H = (int)h;
     for (int dist = 1; dist < (int)max_dist; ++dist) {
@@ -163,13 +167,37 @@ __global__ void k_dilate(const uint8_t* src, uint8_t* dst, const uint8_t* valid,
             bool ok = true;
             for (int j = 0; j < 8 && ok; ++j) ok = valid[(size_t)(cy + oy[j]) * w + (cx + ox[j])] != 0;
             if (!ok) continue;
-            const size_t s = ((size_t)(H - cy - 1) * w + cx) * 3, d = ((size_t)(H - (int)y - 1) * w + x) * 3;
-            dst[d + 0] = src[s + 0];
-            dst[d + 1] = src[s + 1];
-            dst[d + 2] = src[s + 2];
+            centre[p] = (uint32_t)cy * w + (uint32_t)cx;
             return;
         }
     }
+}
+
+// Pass 2: the colour an invalid pixel ends with.  It copies its centre c as
+// the in-place loop finds it: c's final colour when c is an invalid pixel the
+// loop has already passed (only an isolated hole can be a centre, since a
+// centre's eight neighbours are valid), else c's colour before dilation.  The
+// chain moves to ever earlier pixels, so it ends.  dst starts as a copy of src.
+__global__ void k_dilate_fill(const uint8_t* src, uint8_t* dst, const uint8_t* valid, const uint32_t* centre,
+                              uint32_t w, uint32_t h)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const uint32_t p = y * w + x;
+    uint32_t c = centre[p];
+    if (c == kNoCentre) return;
+    uint32_t q = p;
+    while (!valid[c] && c < q) {
+        const uint32_t n = centre[c];
+        if (n == kNoCentre) break;  // c was never filled: its colour before dilation
+        q = c;
+        c = n;
+    }
+    const uint32_t cy = c / w, cx = c - cy * w;
+    const size_t s = ((size_t)(h - cy - 1) * w + cx) * 3, d = ((size_t)(h - y - 1) * w + x) * 3;
+    dst[d + 0] = src[s + 0];
+    dst[d + 1] = src[s + 1];
+    dst[d + 2] = src[s + 2];
 }
 
 }  // namespace

@@ -1847,8 +1847,8 @@ struct Export {
     }
 };
 
-// workspace of any export of a map: 12 bytes per pixel at most, plus the range keys
-size_t export_bytes(const fornos_meshmap* mm) { return 12 * (size_t)mm->width * mm->height + 1024; }
+// workspace of any export of a map: 16 bytes per pixel at most, plus the range keys
+size_t export_bytes(const fornos_meshmap* mm) { return 16 * (size_t)mm->width * mm->height + 2048; }
 
 fornos_status export_check(const fornos_solver* s, uint32_t comps, const char* what)
 {
@@ -1930,9 +1930,13 @@ fornos_status export_finish_rgb(Export& X, const fornos_meshmap* mm, uint8_t* dr
             hipLaunchKernelGGL(k_valid_texels, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st,
                                (const uint32_t*)mm->indices, mm->n, valid);
         }
+        uint32_t* centre = X.take<uint32_t>(npix);
         HIP_TRY(hipMemcpyAsync(out, drgb, npix * 3, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_dilate, dim3(div_up(mm->width, kBlock), mm->height), dim3(kBlock), 0, st,
-                           (const uint8_t*)drgb, out, (const uint8_t*)valid, mm->width, mm->height, (uint32_t)dilate);
+        const dim3 grid(div_up(mm->width, kBlock), mm->height);
+        hipLaunchKernelGGL(k_dilate_centre, grid, dim3(kBlock), 0, st, (const uint8_t*)valid, centre, mm->width,
+                           mm->height, (uint32_t)dilate);
+        hipLaunchKernelGGL(k_dilate_fill, grid, dim3(kBlock), 0, st, (const uint8_t*)drgb, out, (const uint8_t*)valid,
+                           (const uint32_t*)centre, mm->width, mm->height);
         res = out;
     }
     return X.read(rgb, res, npix * 3);

@@ -121,10 +121,11 @@ def vector_planes(values, indices, w, h):
 
 def dilate_rgb(rgb, valid, w, h, max_dist):
     """dilateRGB (Src/image.cpp:127-183) on rgb [w*h, 3] (image rows) and
-    valid [w*h] (map order): an invalid pixel takes the colour of the first
-    centre pos + offset * dist (dist < max_dist) strictly inside the border
-    whose eight neighbours are valid; colours read before any fill."""
-    src = rgb.copy()
+    valid [w*h] (map order), as its loops run on one thread: the pixels in
+    map row-major order (y, then x), each invalid one taking, in place, the
+    colour the image holds at the first centre pos + offset * dist
+    (dist < max_dist) strictly inside the border whose eight neighbours are
+    valid -- so a centre the loop has already filled gives its new colour."""
     out = rgb.copy()
     vm = valid.reshape(h, w)
     # centres whose eight neighbours are valid
@@ -142,7 +143,7 @@ def dilate_rgb(rgb, valid, w, h, max_dist):
             for ox, oy in OFFSETS:
                 cx, cy = x + ox * dist, y + oy * dist
                 if 0 < cx < w - 1 and 0 < cy < h - 1 and good[cy, cx]:
-                    out[(h - y - 1) * w + x] = src[(h - cy - 1) * w + cx]
+                    out[(h - y - 1) * w + x] = out[(h - cy - 1) * w + cx]
                     done = True
                     break
             if done:

@@ -51,6 +51,34 @@ def test_dilation_takes_the_first_fully_surrounded_centre():
     assert np.array_equal(a[2:5, 2:5], b[2:5, 2:5])
 
 
+def test_dilation_fills_in_place_in_row_major_order():
+    # dilateRGB fills the image in place as its loops visit the map's pixels
+    # (y, then x): a centre that is itself a hole the loop has already filled
+    # gives its new colour.  Map rows top to bottom ('#' = texel):
+    mask = ["###.####",
+            ".##..###",
+            ".####.##",
+            "###..###",
+            "########",
+            "########",
+            "#.#.#..#",
+            "###...##"]
+    w = h = 8
+    valid = np.array([[c == "#" for c in row] for row in mask])
+    idx = np.flatnonzero(valid.ravel())
+    vals = (0.2 + 0.7 * np.arange(len(idx)) / len(idx)).astype(np.float32)
+    rgb, _ = ox.float_rgb8(vals, idx, w, h, normalize=False, dilate=4)
+    img = rgb.reshape(h, w, 3)[::-1, :, 0]  # back to map rows
+    # (6, 1) is a hole with eight valid neighbours; its first centre is the
+    # texel (4, 1), two rows up, so it takes that colour
+    assert not valid[6, 1] and valid[5:8, 0:3].sum() == 8
+    assert img[6, 1] == img[4, 1] != 0
+    # (6, 3), later on the same row, finds (6, 1) first (offset (-1, 0) at
+    # distance 2, nothing earlier qualifies): it copies (6, 1)'s new colour,
+    # where a read before any fill would give black
+    assert img[6, 3] == img[4, 1]
+
+
 def test_normal_quantisation_and_rgb_validity():
     n = np.array([[0, 0, 1], [1, 0, 0], [-1, -1, -1], [np.nan, 0, 0]], np.float32)
     rgb = ox.normal_rgb8(n, [0, 1, 2, 3], 2, 2, dilate=0).reshape(2, 2, 3)
