@@ -1484,6 +1484,13 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
             return set_error(FORNOS_UNSUPPORTED, "fornos_batch_create: a job has 2^31 rays or more");
         b->solvers.push_back(s);
     }
+    // longest first (texels x samples x log2 triangles): the persistent waves
+    // start on the big jobs and the small ones fill the last waves' tail
+    auto cost = [](const fornos_solver* s) {
+        return (double)s->mm->n * s->p.sample_count * std::log2(2.0 + s->mm->sc->info.n_triangles);
+    };
+    std::stable_sort(b->solvers.begin(), b->solvers.end(),
+                     [&](const fornos_solver* x, const fornos_solver* y) { return cost(x) > cost(y); });
     HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->solvers[0]->mm->sc->device));
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     if (fornos_status st = dev_alloc(&b->counters, n)) return st;
