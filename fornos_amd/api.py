@@ -276,6 +276,13 @@ class MeshMapping(_Handle):
     def reset(self):
         check(lib().fornos_meshmap_reset(self._h))
 
+    PREDICATE_REFERENCE, PREDICATE_WATERTIGHT = 0, 1
+
+    def set_predicate(self, predicate):
+        """PREDICATE_REFERENCE (default, the reference's triangle test) or
+        PREDICATE_WATERTIGHT (fornos_meshmap_set_predicate)."""
+        check(lib().fornos_meshmap_set_predicate(self._h, int(predicate)))
+
     @property
     def count(self):
         n = C.c_uint32()
@@ -484,7 +491,7 @@ class Batch(_Handle):
 
     def __init__(self, solvers):
         super().__init__()
-        self.solvers = list(solvers)  # the batch holds raw solver handles: keep them alive
+        self.solvers = list(solvers)  # for reading results; the native batch holds its own references
         arr = (C.c_void_p * len(self.solvers))(*[s.handle.value for s in self.solvers])
         check(lib().fornos_batch_create(arr, len(self.solvers), C.byref(self._h)))
 

@@ -100,6 +100,46 @@ __device__ inline bool node_visit(const Ray& r, float4 lo, float4 hi, float T)
     return ray_aabb(r.o, r.d, lo, hi) < T;
 }
 
+// The node test of the watertight mapping: false only when the ray misses the
+// box for certain, so that a ray through a vertex or edge reaches the leaves
+// of every triangle sharing it.  The reference test rejects a box whose face
+// lies in the plane a ray travels in: for d_i = 0 and o_i = lo_i or hi_i the
+// slab distance is 0/0 = NaN and the comparisons fail.  Here such an axis is
+// a containment test (lo_i <= o_i <= hi_i: the ray never leaves the plane),
+// and the other slab distances carry the 8-ulp margin of node_visit (their
+// error is at most 3 units in the last place), so entry <= exit, exit >= 0
+// and entry < T hold whenever they hold for the exact distances.
+__device__ inline bool node_visit_wt(const Ray& r, float4 lo, float4 hi, float T)
+{
+    float a, b;
+    if (!r.slow) {
+        const float t1x = (lo.x - r.o.x) * r.inv.x, t2x = (hi.x - r.o.x) * r.inv.x;
+        const float t1y = (lo.y - r.o.y) * r.inv.y, t2y = (hi.y - r.o.y) * r.inv.y;
+        const float t1z = (lo.z - r.o.z) * r.inv.z, t2z = (hi.z - r.o.z) * r.inv.z;
+        a = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+        b = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+    } else {
+        // components below 2^-40: the quotients themselves, zero components by containment
+        a = -INFINITY;
+        b = INFINITY;
+        const float o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+        const float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (d[i] == 0.0f) {
+                if (o[i] < l[i] || o[i] > h[i]) return false;
+            } else {
+                const float t1 = (l[i] - o[i]) / d[i], t2 = (h[i] - o[i]) / d[i];
+                a = fmaxf(a, fminf(t1, t2));
+                b = fminf(b, fmaxf(t1, t2));
+            }
+        }
+    }
+    const float al = a - __builtin_fmaf(fabsf(a), kSlabRel, kSlabAbs);
+    const float bh = b + __builtin_fmaf(fabsf(b), kSlabRel, kSlabAbs);
+    return bh >= 0.0f && al <= bh && al < T;
+}
+
 // Conservative form of the same test for the culling tree: false only when
 // the box is missed for certain.  The culling boxes contain the reference
 // leaf boxes grown by pad = 2^-16 * max(1, max |coordinate|), which moves each

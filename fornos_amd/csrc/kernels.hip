@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <chrono>
@@ -43,7 +44,24 @@ using namespace fornos;
 constexpr float kSplitCost = 1.0f;
 constexpr double kSplitPredict = 0.95, kSplitKeep = 0.9;
 
+// Shared ownership across the C ABI, as the reference's shared_ptr members (a
+// solver holds its MeshMapping, Src/solver_ao.h:93): an object made from
+// another holds a reference to it, *_destroy drops the caller's, and the last
+// reference frees it.  So handles may be destroyed in any order -- a garbage
+// collector may finalize a scene before the mapping that uses it.
+template <class T>
+void retain(T* p)
+{
+    if (p) p->refs.fetch_add(1, std::memory_order_relaxed);
+}
+template <class T>
+void release(T* p)
+{
+    if (p && p->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete p;
+}
+
 struct fornos_scene {
+    std::atomic<int> refs{1};
     int device = 0;
     uint32_t cus = 256;  // compute units of the scene's device
     hipStream_t stream = nullptr;
@@ -79,11 +97,13 @@ struct fornos_scene {
 };
 
 struct fornos_meshmap {
-    fornos_scene* sc = nullptr;
+    std::atomic<int> refs{1};
+    fornos_scene* sc = nullptr;  // one reference held
     uint32_t n = 0, width = 0, height = 0;
     uint32_t work_count = 0;  // n rounded up to 64, Src/meshmapping.cpp:165
     uint32_t offset = 0;      // texels mapped so far
     int cull = 1;
+    int predicate = FORNOS_PREDICATE_REFERENCE;  // triangle test of the mapping
     Marks marks;              // parts of the last fornos_meshmap_submit
     uint32_t mark_from = 0;   // texel its first part started at
     float4 *pix = nullptr, *pixt = nullptr, *coords = nullptr;
@@ -94,13 +114,15 @@ struct fornos_meshmap {
     {
         if (sc && sc->stream) (void)hipStreamSynchronize(sc->stream);
         dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx); dev_free(indices); dev_free(order);
+        release(sc);
     }
 };
 
 struct fornos_solver {
+    std::atomic<int> refs{1};
     int kind = 0;
     fornos_solver_params p{};
-    fornos_meshmap* mm = nullptr;
+    fornos_meshmap* mm = nullptr;  // one reference held
     uint32_t P = 0;
     uint64_t work_total = 0, offset = 0;  // work items (rays or texels)
     float4* samples = nullptr;      // sample table, reference order (reductions)
@@ -137,6 +159,7 @@ struct fornos_solver {
         dev_free(raybuf); dev_free(counter);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (done_ev) (void)hipEventDestroy(done_ev);
+        release(mm);
     }
     bool hemi() const { return kind >= FORNOS_SOLVER_AO; }
     uint32_t comps() const
@@ -674,7 +697,7 @@ fornos_status fornos_scene_export_soup(const fornos_scene* sc, float* p4, float*
     return FORNOS_OK;
 }
 
-void fornos_scene_destroy(fornos_scene* sc) { delete sc; }
+void fornos_scene_destroy(fornos_scene* sc) { release(sc); }
 
 fornos_status fornos_meshmap_create(fornos_scene* sc, const fornos_texel_view* tv, int cull,
                                     fornos_meshmap** out)
@@ -685,6 +708,7 @@ fornos_status fornos_meshmap_create(fornos_scene* sc, const fornos_texel_view* t
         return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_create: texel positions/directions missing");
     std::unique_ptr<fornos_meshmap> mm(new fornos_meshmap());
     mm->sc = sc;
+    retain(sc);
     mm->n = tv->count;
     mm->width = tv->width;
     mm->height = tv->height;
@@ -778,6 +802,7 @@ fornos_status fornos_meshmap_create_from_mesh_hybrid(fornos_scene* sc, const for
     }
     std::unique_ptr<fornos_meshmap> mm(new fornos_meshmap());
     mm->sc = sc;
+    retain(sc);
     mm->width = W;
     mm->height = H;
     mm->cull = cull ? 1 : 0;
@@ -867,10 +892,16 @@ static fornos_status meshmap_launch(fornos_meshmap* mm, uint32_t t0, uint32_t t1
     if (t1 <= t0) return FORNOS_OK;
     mm->order_valid = false;
     const uint32_t blocks = div_up(t1 - t0, kBlock);
-    if (mm->cull)
-        hipLaunchKernelGGL(k_meshmap<true>, dim3(blocks), dim3(kBlock), 0, mm->sc->stream, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
+    hipStream_t st = mm->sc->stream;
+    const bool wt = mm->predicate == FORNOS_PREDICATE_WATERTIGHT;
+    if (mm->cull && wt)
+        hipLaunchKernelGGL((k_meshmap<true, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
+    else if (mm->cull)
+        hipLaunchKernelGGL((k_meshmap<true, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
+    else if (wt)
+        hipLaunchKernelGGL((k_meshmap<false, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
     else
-        hipLaunchKernelGGL(k_meshmap<false>, dim3(blocks), dim3(kBlock), 0, mm->sc->stream, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
+        hipLaunchKernelGGL((k_meshmap<false, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
     HIP_TRY(hipGetLastError());
     return FORNOS_OK;
 }
@@ -896,6 +927,14 @@ fornos_status fornos_meshmap_run_all(fornos_meshmap* mm, float* ms)
     if (fornos_status st = meshmap_launch(mm, std::min(mm->offset, mm->n), mm->n)) return st;
     mm->offset = mm->work_count;
     return tm.stop(ms);
+}
+
+fornos_status fornos_meshmap_set_predicate(fornos_meshmap* mm, int predicate)
+{
+    if (!mm || (predicate != FORNOS_PREDICATE_REFERENCE && predicate != FORNOS_PREDICATE_WATERTIGHT))
+        return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_set_predicate: FORNOS_PREDICATE_REFERENCE or _WATERTIGHT");
+    mm->predicate = predicate;
+    return FORNOS_OK;
 }
 
 fornos_status fornos_meshmap_reset(fornos_meshmap* mm)
@@ -973,7 +1012,7 @@ fornos_status fornos_meshmap_texel_count(const fornos_meshmap* mm, uint32_t* out
     return FORNOS_OK;
 }
 
-void fornos_meshmap_destroy(fornos_meshmap* mm) { delete mm; }
+void fornos_meshmap_destroy(fornos_meshmap* mm) { release(mm); }
 
 fornos_status fornos_solver_create(int kind, const fornos_solver_params* params, fornos_meshmap* mm,
                                    const float* table, uint32_t n_perm, fornos_solver** out)
@@ -986,6 +1025,7 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
     s->kind = kind;
     s->p = *params;
     s->mm = mm;
+    retain(mm);
     if ((kind == FORNOS_SOLVER_NORMALS || kind == FORNOS_SOLVER_BENT_NORMALS) && params->tangent_space && !mm->pixt)
         return set_error(FORNOS_INVALID_ARG, "fornos_solver_create: tangent space needs per-texel normals, tangents and bitangents");
     if (s->hemi()) {
@@ -1397,12 +1437,12 @@ fornos_status fornos_solver_get_launch(const fornos_solver* s, uint32_t* blocks_
     return FORNOS_OK;
 }
 
-void fornos_solver_destroy(fornos_solver* s) { delete s; }
+void fornos_solver_destroy(fornos_solver* s) { release(s); }
 
 }  // extern "C"
 
 struct fornos_batch {
-    std::vector<fornos_solver*> solvers;
+    std::vector<fornos_solver*> solvers;  // one reference held on each
     std::vector<RayArgs> jobs;
     hipStream_t stream = nullptr;
     RayArgs* djobs = nullptr;
@@ -1419,6 +1459,7 @@ struct fornos_batch {
         dev_free(djobs);
         dev_free(counters);
         if (stream) (void)hipStreamDestroy(stream);
+        for (fornos_solver* s : solvers) release(s);
     }
 };
 
@@ -1483,6 +1524,7 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         if ((uint64_t)s->mm->n * s->p.sample_count >= (1ull << 31))
             return set_error(FORNOS_UNSUPPORTED, "fornos_batch_create: a job has 2^31 rays or more");
         b->solvers.push_back(s);
+        retain(s);
     }
     // longest first (texels x samples x log2 triangles): the persistent waves
     // start on the big jobs and the small ones fill the last waves' tail

@@ -48,13 +48,66 @@ __device__ inline bool map_tri(V3 o, V3 d, uint32_t k, const DevScene& sc, float
     return false;
 }
 
+// The same triangle with a watertight containment test in place of the
+// reference's plane distance + fp64 barycentrics with their -1e-5 tolerance:
+// the ray/triangle method of Woop, Benthin and Wald ("Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013).  The vertices, relative to the origin, are
+// sheared into the ray's frame (z along the dominant direction component),
+// the three 2-D edge functions U, V, W are evaluated there (recomputed in
+// double when one is exactly zero), and the ray hits iff they share a sign:
+// a ray through an edge or vertex shared by two triangles hits at least one
+// of them.  t = (U Az + V Bz + W Cz) / (U + V + W); the barycentrics are
+// (U, V, W) / (U + V + W).  The side rule (culling) is the reference's n.d
+// test.  No reference counterpart: the production option of the mapping.
+template <int MODE>
+__device__ inline bool map_tri_wt(V3 o, V3 d, uint32_t k, const DevScene& sc, float tmax, float& t_out, V3& bc)
+{
+    const float4 q0 = sc.tri[k];
+    const float nd = dot(d, xyz(q0));
+    const bool side = MODE == 0 ? (fabsf(nd) > 0) : (MODE == 1 ? (nd > 0) : (nd < 0));
+    if (!side) return false;
+    const float dd[3] = {d.x, d.y, d.z};
+    const float ad[3] = {fabsf(d.x), fabsf(d.y), fabsf(d.z)};
+    const int kz = ad[0] > ad[1] ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    if (dd[kz] < 0.0f) {  // keep the winding
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    const float sx = dd[kx] / dd[kz], sy = dd[ky] / dd[kz], sz = 1.0f / dd[kz];
+    const float4 a4 = sc.spos[3 * k], b4 = sc.spos[3 * k + 1], c4 = sc.spos[3 * k + 2];
+    const float A[3] = {a4.x - o.x, a4.y - o.y, a4.z - o.z};
+    const float B[3] = {b4.x - o.x, b4.y - o.y, b4.z - o.z};
+    const float C[3] = {c4.x - o.x, c4.y - o.y, c4.z - o.z};
+    const float ax = A[kx] - sx * A[kz], ay = A[ky] - sy * A[kz];
+    const float bx = B[kx] - sx * B[kz], by = B[ky] - sy * B[kz];
+    const float cx = C[kx] - sx * C[kz], cy = C[ky] - sy * C[kz];
+    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)cx * by - (double)cy * bx);
+        V = (float)((double)ax * cy - (double)ay * cx);
+        W = (float)((double)bx * ay - (double)by * ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (sz * A[kz]) + V * (sz * B[kz]) + W * (sz * C[kz]);
+    const float t = T / det;
+    if (!(t >= 0 && t < tmax)) return false;
+    t_out = t;
+    bc = mk(U / det, V / det, W / det);
+    return true;
+}
+
 // One pass of raycastBVH / raycastBackBVH in the reference's stackless DFS
 // order (Shaders/meshmapping_nobackfaces.comp:102-121): visit a node iff its
 // slab entry is < the running best, else jump past its subtree.  The node
 // test is node_visit: the reference's RayAABB(...) < cur decided on the
 // reciprocal-multiply slab distances wherever they leave no doubt, with the
-// reference's divisions for the rest.
-template <int MODE>
+// reference's divisions for the rest.  WT: the conservative node_visit_wt and
+// the watertight triangle test.
+template <int MODE, bool WT>
 __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, uint32_t& idx, V3& bc)
 {
     const V3 o = r.o, d = r.d;
@@ -62,13 +115,14 @@ __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, ui
     while (i < sc.n_nodes) {
         const float4 lo = sc.nodes[2 * i], hi = sc.nodes[2 * i + 1];
         const uint32_t w = __float_as_uint(hi.w);
-        if (node_visit(r, lo, hi, cur)) {
+        if (WT ? node_visit_wt(r, lo, hi, cur) : node_visit(r, lo, hi, cur)) {
             if (w & kLeafFlag) {
                 const uint32_t first = __float_as_uint(lo.w), end = first + (w & ~kLeafFlag);
                 for (uint32_t k = first; k < end; ++k) {
                     float t;
                     V3 b;
-                    if (map_tri<MODE>(o, d, k, sc, cur, t, b)) { cur = t; idx = 3 * k; bc = b; }
+                    const bool hit = WT ? map_tri_wt<MODE>(o, d, k, sc, cur, t, b) : map_tri<MODE>(o, d, k, sc, cur, t, b);
+                    if (hit) { cur = t; idx = 3 * k; bc = b; }
                 }
             }
             ++i;
@@ -80,7 +134,9 @@ __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, ui
 
 // Eight waves per SIMD (57 VGPRs, no spill; the compiler's own choice of 66
 // leaves room for 7): more texels in flight hide the dependent node loads.
-template <bool CULL>
+// WT: the watertight node and triangle tests (node_visit_wt, map_tri_wt)
+// instead of the reference's.
+template <bool CULL, bool WT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_meshmap(const float4* __restrict__ pix, uint32_t begin,
                                                      uint32_t end, DevScene sc,
                                                      float4* __restrict__ coords,
@@ -102,12 +158,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     fw.slow = bw.slow = !(fabsf(d.x) >= kSlowDir && fabsf(d.y) >= kSlowDir && fabsf(d.z) >= kSlowDir);
     if (CULL) {
         // meshmapping_nobackfaces.comp:182-199: one running t shared by both passes
-        map_pass<1>(fw, sc, t, idx, bc);
-        map_pass<2>(bw, sc, t, idx, bc);
+        map_pass<1, WT>(fw, sc, t, idx, bc);
+        map_pass<2, WT>(bw, sc, t, idx, bc);
     } else {
         // meshmapping.comp:139-162: the backward pass starts from the forward t
-        map_pass<0>(fw, sc, t, idx, bc);
-        map_pass<0>(bw, sc, t, idx, bc);
+        map_pass<0, WT>(fw, sc, t, idx, bc);
+        map_pass<0, WT>(bw, sc, t, idx, bc);
     }
     coords[g] = make_float4(t, bc.x, bc.y, bc.z);
     tidx[g] = idx;

@@ -11,6 +11,12 @@
  * Threading: a handle's work is queued on one HIP stream owned by its scene;
  * calls on one handle (and on handles sharing a scene) must be serialised by
  * the caller.  Distinct scenes may be driven from distinct threads.
+ *
+ * Ownership: as the reference's shared_ptr members (a solver holds its
+ * MeshMapping, Src/solver_ao.h:93), a mapping holds a reference to its
+ * scene, a solver to its mapping and a batch to its solvers.  *_destroy
+ * drops the caller's reference; an object is freed when its last reference
+ * goes, so handles may be destroyed in any order.
  */
 #ifndef FORNOS_HIP_H
 #define FORNOS_HIP_H
@@ -162,6 +168,20 @@ fornos_status fornos_meshmap_run_all(fornos_meshmap *mm, float *ms);
 /* Rewinds progress to 0 so the next run_step/run_all maps every texel again
  * (the reference re-runs a pass only through a new init). */
 fornos_status fornos_meshmap_reset(fornos_meshmap *mm);
+/* Triangle test of the mapping.  FORNOS_PREDICATE_REFERENCE (default): the
+ * reference's plane distance and fp64 barycentrics accepting b >= -1e-5
+ * (Shaders/meshmapping.comp:67-86, meshmapping_nobackfaces.comp:60-100),
+ * whose results the oracle checks bit for bit.  FORNOS_PREDICATE_WATERTIGHT:
+ * the watertight test of Woop, Benthin and Wald (JCGT 2013): a ray through a
+ * shared edge or vertex hits at least one of its triangles; same side
+ * (culling) rule, same traversal order with a conservative box test (the
+ * reference's rejects a box whose face lies in the plane that a ray with
+ * a zero direction component travels in).  Hit ids differ from the
+ * reference's only for rays passing near a triangle edge.  Takes effect at
+ * the next run. */
+#define FORNOS_PREDICATE_REFERENCE 0
+#define FORNOS_PREDICATE_WATERTIGHT 1
+fornos_status fornos_meshmap_set_predicate(fornos_meshmap *mm, int predicate);
 /* Asynchronous submission of the remaining mapping work as `parts` launches
  * (1..4096) with a completion mark after each; returns without waiting.
  * Replaces the reference's one-runStep-per-frame pacing
@@ -337,8 +357,8 @@ void fornos_solver_destroy(fornos_solver *s);
  * counterpart: the reference runs one solver step per UI frame
  * (Src/fornos.cpp:239-251); each job's results equal its solver's
  * fornos_solver_run_all bit for bit and are read with fornos_solver_read.
- * The solvers must stay alive while the batch exists; their own streams must
- * be idle when fornos_batch_run is called. */
+ * The batch holds a reference to each solver; their own streams must be
+ * idle when fornos_batch_run is called. */
 typedef struct fornos_batch fornos_batch;
 fornos_status fornos_batch_create(fornos_solver *const *solvers, uint32_t n, fornos_batch **out);
 /* Launch shape, as fornos_solver_set_launch. */
