@@ -92,7 +92,9 @@ def ao_pass(fa, workload, steps, warmup, log):
            "wall": wall, "steps": steps, "ms_per_step": 1000.0 * wall / steps,
            "value": live * S * steps / wall, "launches": launches,
            "launch_ms": float(np.mean(rays_ms)) / launches, "reduce_ms": float(np.mean(reduce_ms)),
-           "pass_kernel_ms": float(np.mean(pass_ms))}
+           "pass_kernel_ms": float(np.mean(pass_ms)),
+           "ray_order": ("Morton code of the texels' hit positions"
+                         if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel set order")}
     log("# %s AO-%d: %.2f ms/step, %.3e rays/s" % (workload, S, res["ms_per_step"], res["value"]))
     return res
 
@@ -290,6 +292,7 @@ def main():
                       % (k["cfg"]["size"], k["info"]["n_triangles"], k["S"], k["params"]["minDistance"],
                          k["params"]["maxDistance"]),
             "texels": k["texels"], "rays_per_step": k["rays_per_step"], "mapping_ms": k["map_ms"],
+            "ray_order": k["ray_order"],
             "roofline": roofline(k)}
         del k
 
@@ -323,7 +326,7 @@ def main():
                                   p["maxDistance"]),
                    "texels": head["texels"], "rays_per_step": head["rays_per_step"],
                    "bvh": "reference binary tree (exact reachability) + 4-wide SAH culling tree",
-                   "ray_order": "Morton code of the texels' hit positions"},
+                   "ray_order": head["ray_order"]},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity, "mapping_ms": head["map_ms"],
         "texels_per_s": head["live"] / (head["ms_per_step"] * 1e-3),
         "workloads": workloads, "batch": batch, "bake": bake, "denoise": denoise,

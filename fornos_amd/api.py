@@ -421,6 +421,12 @@ class Solver(_Handle):
         check(lib().fornos_solver_get_ray_order(self._h, C.byref(t)))
         return t.value
 
+    def effective_ray_order(self):
+        """RAY_ORDER_TEXELS or RAY_ORDER_MORTON: what the next pass uses."""
+        t = C.c_int()
+        check(lib().fornos_solver_effective_ray_order(self._h, C.byref(t)))
+        return t.value
+
 
 class HeightSolver(Solver):
     kind = _abi.SOLVER_HEIGHT

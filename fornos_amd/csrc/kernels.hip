@@ -482,14 +482,21 @@ float scene_diagonal(const fornos_scene* sc)
 // paired pass is 3 % slower in Morton order).
 constexpr float kMortonRayReach = 0.25f;
 
+// FORNOS_RAY_ORDER_TEXELS or _MORTON: the order a solver's passes use.
+int effective_order(const fornos_solver* s)
+{
+    if (!s->hemi() || s->ray_order == FORNOS_RAY_ORDER_TEXELS) return FORNOS_RAY_ORDER_TEXELS;
+    if (s->ray_order == FORNOS_RAY_ORDER_AUTO && !(s->p.max_distance >= kMortonRayReach * scene_diagonal(s->mm->sc)))
+        return FORNOS_RAY_ORDER_TEXELS;
+    return FORNOS_RAY_ORDER_MORTON;
+}
+
 // The order a solver's passes hand out their texels in (null: the texel
 // set's own order).
 fornos_status pass_order(fornos_solver* s, const uint32_t** list)
 {
     *list = nullptr;
-    if (!s->hemi() || s->ray_order == FORNOS_RAY_ORDER_TEXELS) return FORNOS_OK;
-    if (s->ray_order == FORNOS_RAY_ORDER_AUTO && !(s->p.max_distance >= kMortonRayReach * scene_diagonal(s->mm->sc)))
-        return FORNOS_OK;
+    if (effective_order(s) == FORNOS_RAY_ORDER_TEXELS) return FORNOS_OK;
     return meshmap_order(s->mm, list);
 }
 
@@ -1424,6 +1431,13 @@ fornos_status fornos_solver_get_ray_order(const fornos_solver* s, int* order)
 {
     if (!s || !order) return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_ray_order: null argument");
     *order = s->ray_order;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_effective_ray_order(const fornos_solver* s, int* order)
+{
+    if (!s || !order) return set_error(FORNOS_INVALID_ARG, "fornos_solver_effective_ray_order: null argument");
+    *order = effective_order(s);
     return FORNOS_OK;
 }
 

@@ -348,6 +348,9 @@ fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_
 #define FORNOS_RAY_ORDER_AUTO 2
 fornos_status fornos_solver_set_ray_order(fornos_solver *s, int order);
 fornos_status fornos_solver_get_ray_order(const fornos_solver *s, int *order);
+/* The order the next pass will use: FORNOS_RAY_ORDER_TEXELS or _MORTON
+ * (FORNOS_RAY_ORDER_AUTO resolved for this solver's maxDistance and scene). */
+fornos_status fornos_solver_effective_ray_order(const fornos_solver *s, int *order);
 void fornos_solver_destroy(fornos_solver *s);
 
 /* ---- batched jobs -------------------------------------------------------- */

@@ -168,12 +168,20 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
            "thickness": fa.ThicknessSolver}[kind]
     s = cls(p["sampleCount"], p["minDistance"], p["maxDistance"]).init(field.prep["texels"], field.mm)
     assert s.get_launch() == (8, 48, 16) and s.get_ray_order() == s.RAY_ORDER_AUTO
+    # AUTO: Morton order for rays at least a quarter of the scene box's diagonal long
+    nodes, _ = field.scene.reference_nodes()
+    box = nodes[0, :6].view(np.float32).astype(np.float64)
+    long_rays = p["maxDistance"] >= 0.25 * np.float32(np.linalg.norm(box[3:] - box[:3]))
+    assert s.effective_ray_order() == (s.RAY_ORDER_MORTON if long_rays else s.RAY_ORDER_TEXELS)
     s.run_all()
     ref = s.getResults().view(np.uint32).copy()
     live = int(np.sum(field.tidx != MISS))
     for k, shape in enumerate([(8, 16, 24), (8, 16, 32), (1, 1, 16), (3, 64, 24), (16, 7, 32)]):
         s.set_launch(*shape)
         s.set_ray_order((s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON, s.RAY_ORDER_AUTO)[k % 3])
+        assert s.effective_ray_order() in (s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON)
+        if k % 3 < 2:
+            assert s.effective_ray_order() == k % 3
         assert s.get_launch() == shape
         s.reset()
         s.run_all()
