@@ -283,6 +283,22 @@ class MeshMapping(_Handle):
         PREDICATE_WATERTIGHT (fornos_meshmap_set_predicate)."""
         check(lib().fornos_meshmap_set_predicate(self._h, int(predicate)))
 
+    TRAVERSAL_WIDE, TRAVERSAL_REFERENCE = 0, 1
+
+    def set_traversal(self, traversal):
+        """TRAVERSAL_WIDE (default) or TRAVERSAL_REFERENCE: the tree the
+        reference predicate's passes walk; results are identical
+        (fornos_meshmap_set_traversal)."""
+        check(lib().fornos_meshmap_set_traversal(self._h, int(traversal)))
+
+    def redo_counts(self):
+        """{"undecided": passes redone on the reference tree for a hit whose
+        reachability the wide traversal could not decide, "overflow": for a
+        full stack}, since creation or the last reset."""
+        u, o = C.c_uint32(), C.c_uint32()
+        check(lib().fornos_meshmap_redo_counts(self._h, C.byref(u), C.byref(o)))
+        return {"undecided": u.value, "overflow": o.value}
+
     @property
     def count(self):
         n = C.c_uint32()

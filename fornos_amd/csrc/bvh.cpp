@@ -539,14 +539,30 @@ void build_cull_bvh(const FlatBVH& ref, const std::vector<float>& spos4, float p
                         continue;
                     }
                 }
+                double edge = 0;  // longest triangle edge of the leaf
                 for (uint32_t k = 3 * first; k < 3 * (first + cnt); ++k) {
                     const V3 p = mk(spos4[4 * k], spos4[4 * k + 1], spos4[4 * k + 2]);
                     lo = hmin(lo, p);
                     hi = hmax(hi, p);
+                    const uint32_t k1 = k % 3 == 2 ? k - 2 : k + 1;
+                    double e2 = 0;
+                    for (int a = 0; a < 3; ++a) {
+                        const double de = (double)spos4[4 * k + a] - spos4[4 * k1 + a];
+                        e2 += de * de;
+                    }
+                    edge = std::max(edge, std::sqrt(e2));
                 }
+                // The mapping accepts barycentrics down to -1e-5
+                // (Shaders/meshmapping.comp:75-81): its hits lie in the
+                // triangle scaled by 1 + 3e-5 about its centroid, within
+                // 3e-5 x the longest edge of the triangle.  With this margin
+                // the box still holds them, so a ray enters it before any
+                // such hit, and a traversal that prunes boxes entered at or
+                // beyond its best hit keeps every leaf that could still win.
+                const float m = pad + (float)(edge * (1.0 / 16384.0));
                 CullPrim cp;
-                cp.lo = lo - mk(pad, pad, pad);
-                cp.hi = hi + mk(pad, pad, pad);
+                cp.lo = lo - mk(m, m, m);
+                cp.hi = hi + mk(m, m, m);
                 cp.c = (cp.lo + cp.hi) * 0.5f;
                 cp.ref = i;
                 out_c.push_back(cp);

@@ -104,6 +104,8 @@ struct fornos_meshmap {
     uint32_t offset = 0;      // texels mapped so far
     int cull = 1;
     int predicate = FORNOS_PREDICATE_REFERENCE;  // triangle test of the mapping
+    int traversal = FORNOS_TRAVERSAL_WIDE;        // tree the reference predicate's passes walk
+    uint32_t* redo = nullptr;  // passes redone on the reference tree: undecidable hit, stack overflow
     Marks marks;              // parts of the last fornos_meshmap_submit
     uint32_t mark_from = 0;   // texel its first part started at
     float4 *pix = nullptr, *pixt = nullptr, *coords = nullptr;
@@ -114,6 +116,7 @@ struct fornos_meshmap {
     {
         if (sc && sc->stream) (void)hipStreamSynchronize(sc->stream);
         dev_free(pix); dev_free(pixt); dev_free(coords); dev_free(tidx); dev_free(indices); dev_free(order);
+        dev_free(redo);
         release(sc);
     }
 };
@@ -900,15 +903,28 @@ static fornos_status meshmap_launch(fornos_meshmap* mm, uint32_t t0, uint32_t t1
     mm->order_valid = false;
     const uint32_t blocks = div_up(t1 - t0, kBlock);
     hipStream_t st = mm->sc->stream;
+    const DevScene& d = mm->sc->d;
     const bool wt = mm->predicate == FORNOS_PREDICATE_WATERTIGHT;
-    if (mm->cull && wt)
-        hipLaunchKernelGGL((k_meshmap<true, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
-    else if (mm->cull)
-        hipLaunchKernelGGL((k_meshmap<true, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
-    else if (wt)
-        hipLaunchKernelGGL((k_meshmap<false, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
-    else
-        hipLaunchKernelGGL((k_meshmap<false, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, mm->sc->d, mm->coords, mm->tidx);
+    if (!mm->redo) {
+        if (fornos_status e = dev_alloc(&mm->redo, 2)) return e;
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, st, mm->redo, (size_t)2, 0u);
+    }
+    if (!wt && mm->traversal == FORNOS_TRAVERSAL_WIDE && d.n_cnodes > 0) {
+        if (mm->cull)
+            hipLaunchKernelGGL((k_meshmap_wide<true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords,
+                               mm->tidx, mm->redo);
+        else
+            hipLaunchKernelGGL((k_meshmap_wide<false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords,
+                               mm->tidx, mm->redo);
+    } else if (mm->cull && wt) {
+        hipLaunchKernelGGL((k_meshmap<true, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords, mm->tidx);
+    } else if (mm->cull) {
+        hipLaunchKernelGGL((k_meshmap<true, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords, mm->tidx);
+    } else if (wt) {
+        hipLaunchKernelGGL((k_meshmap<false, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords, mm->tidx);
+    } else {
+        hipLaunchKernelGGL((k_meshmap<false, false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords, mm->tidx);
+    }
     HIP_TRY(hipGetLastError());
     return FORNOS_OK;
 }
@@ -949,6 +965,29 @@ fornos_status fornos_meshmap_reset(fornos_meshmap* mm)
     if (!mm) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_reset: null handle");
     mm->offset = 0;
     mm->marks.clear();
+    if (mm->redo) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, mm->sc->stream, mm->redo, (size_t)2, 0u);
+        HIP_TRY(hipGetLastError());
+    }
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_set_traversal(fornos_meshmap* mm, int traversal)
+{
+    if (!mm || (traversal != FORNOS_TRAVERSAL_WIDE && traversal != FORNOS_TRAVERSAL_REFERENCE))
+        return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_set_traversal: FORNOS_TRAVERSAL_WIDE or _REFERENCE");
+    mm->traversal = traversal;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_meshmap_redo_counts(fornos_meshmap* mm, uint32_t* undecided, uint32_t* overflow)
+{
+    if (!mm || !undecided || !overflow) return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_redo_counts: null argument");
+    uint32_t h[2] = {0, 0};
+    HIP_TRY(hipStreamSynchronize(mm->sc->stream));
+    if (mm->redo) HIP_TRY(hipMemcpy(h, mm->redo, sizeof(h), hipMemcpyDeviceToHost));
+    *undecided = h[0];
+    *overflow = h[1];
     return FORNOS_OK;
 }
 

@@ -132,6 +132,152 @@ __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, ui
     }
 }
 
+// One pass on the 4-wide culling tree with the reference pass's result.  The
+// reference pass (map_pass) keeps, among the triangles of the leaves it
+// reaches, the first in its DFS visit order with the smallest t; it reaches a
+// leaf iff every node of the leaf's chain has slab entry < its running best
+// when tested, and that running best starts at `cur0` (FLT_MAX, or the
+// forward pass's result) and never drops below the pass's result t*.  The
+// soup holds the triangles in the DFS order of their leaves, so the first
+// visited is the one with the smallest soup index.  Here:
+//  - a box is skipped only when the ray enters it no earlier than the best hit
+//    so far: every hit the mapping's tolerance accepts lies strictly inside
+//    its culling box (the boxes' margin), so a skipped box holds no hit that
+//    beats or ties the best;
+//  - a hit that beats the best, or ties it with a smaller index (within this
+//    pass), is taken when its leaf's chain passes with bound t.  Taken hits
+//    never beat the reference's: a taken t < t* would have a chain passing
+//    with t < t* <= the reference's running best, so the reference would
+//    reach the leaf and find it.  And the reference's own winner is taken
+//    whenever its chain passes with its t;
+//  - when the chain passes only with a larger bound (up to cur0), whether the
+//    reference reached the leaf depends on its visit order: the pass is redone
+//    on the reference tree, as when the stack would overflow.
+// Returns 0 when the pass is decided, kMapRedoChain + 1 or kMapRedoStack + 1
+// to redo it (cur, idx and bc are then partial; the caller restores them).
+enum { kMapRedoChain = 0, kMapRedoStack = 1 };  // counters of passes redone on the reference tree
+template <int MODE, int STK>
+__device__ inline int map_wide_pass(const Ray& r, const DevScene& sc, uint32_t* stk, float cur0, float& cur,
+                                    uint32_t& idx, V3& bc)
+{
+    const uint32_t tid = threadIdx.x;
+    bool mine = false;  // the best hit is this pass's own
+    uint32_t sp = 0, node = 0;
+    for (;;) {
+        if (node == kNoChild) {
+            if (sp == 0) return 0;
+            --sp;
+            node = stk[sp * kBlock + tid];
+        }
+        if (node & kLeafFlag) {
+            const uint4 lf = sc.wleaf[node & ~kLeafFlag];
+            node = kNoChild;
+            for (uint32_t k = lf.x; k < lf.x + lf.y; ++k) {
+                float t;
+                V3 b;
+                // ties: a smaller index of this pass wins, so let t == cur
+                // through (cur >= 0 here: the next float up is one ulp above)
+                const float tmax = mine ? __uint_as_float(__float_as_uint(cur) + 1u) : cur;
+                if (!map_tri<MODE>(r.o, r.d, k, sc, tmax, t, b)) continue;
+                if (t == cur && !(3 * k < idx)) continue;
+                if (chain_ok(r, sc, lf.z, t)) {
+                    cur = t;
+                    idx = 3 * k;
+                    bc = b;
+                    mine = true;
+                } else if (chain_ok(r, sc, lf.z, cur0)) {
+                    return kMapRedoChain + 1;
+                }
+            }
+            continue;
+        }
+        const float4* nd = sc.wide + 8 * (size_t)node;
+        const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+        const uint4 ch = *(const uint4*)(nd + 6);
+        const float4 lo[4] = {make_float4(lx.x, ly.x, lz.x, 0), make_float4(lx.y, ly.y, lz.y, 0),
+                              make_float4(lx.z, ly.z, lz.z, 0), make_float4(lx.w, ly.w, lz.w, 0)};
+        const float4 hi[4] = {make_float4(hx.x, hy.x, hz.x, 0), make_float4(hx.y, hy.y, hz.y, 0),
+                              make_float4(hx.z, hy.z, hz.z, 0), make_float4(hx.w, hy.w, hz.w, 0)};
+        const uint32_t chs[4] = {ch.x, ch.y, ch.z, ch.w};
+        float e[4];
+        uint32_t c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float al = 0;
+            const bool pass = (chs[k] != kNoChild) & box_maybe(r, lo[k], hi[k], cur, al);
+            e[k] = pass ? al : INFINITY;
+            c[k] = pass ? chs[k] : kNoChild;
+        }
+#define FORNOS_CSWAP(i, j)                                            \
+    if (e[j] < e[i]) {                                                \
+        const float te = e[i]; e[i] = e[j]; e[j] = te;                \
+        const uint32_t tc = c[i]; c[i] = c[j]; c[j] = tc;             \
+    }
+        FORNOS_CSWAP(0, 1) FORNOS_CSWAP(2, 3) FORNOS_CSWAP(0, 2) FORNOS_CSWAP(1, 3) FORNOS_CSWAP(1, 2)
+#undef FORNOS_CSWAP
+        node = c[0];
+        const uint32_t n = (uint32_t)(c[1] != kNoChild) + (uint32_t)(c[2] != kNoChild) + (uint32_t)(c[3] != kNoChild);
+        if (sp + n > STK) return kMapRedoStack + 1;
+#pragma unroll
+        for (uint32_t k = 1; k <= 3; ++k)
+            if (k <= n) stk[(sp + n - k) * kBlock + tid] = c[k];
+        sp += n;
+    }
+}
+
+constexpr int kMapStack = 24;  // 4-wide mapping stack entries per lane (LDS)
+
+// The reference predicate on the 4-wide culling tree (map_wide_pass), each
+// pass redone on the reference tree when it cannot be decided there, and rays
+// with a direction component below 2^-40 traced on the reference tree from
+// the start.  Results equal k_meshmap<CULL, false>'s bit for bit.
+template <bool CULL>
+__global__ __launch_bounds__(kBlock) void k_meshmap_wide(const float4* __restrict__ pix, uint32_t begin, uint32_t end,
+                                                        DevScene sc, float4* __restrict__ coords,
+                                                        uint32_t* __restrict__ tidx, uint32_t* __restrict__ redo)
+{
+    __shared__ uint32_t stk[kMapStack * kBlock];
+    const uint32_t g = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (g >= end) return;
+    const V3 p = xyz(pix[2 * g]), d = xyz(pix[2 * g + 1]);
+    uint32_t idx = kMiss;
+    V3 bc = mk(0, 0, 0);
+    float t = FLT_MAX;
+    Ray fw, bw;
+    fw.o = bw.o = p;
+    fw.d = d;
+    bw.d = -d;
+    fw.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    bw.inv = -fw.inv;  // 1/(-x) = -(1/x) exactly
+    fw.oi = mk(p.x * fw.inv.x, p.y * fw.inv.y, p.z * fw.inv.z);
+    bw.oi = -fw.oi;    // RN(o * (-inv)) = -RN(o * inv)
+    fw.slow = bw.slow = !(fabsf(d.x) >= kSlowDir && fabsf(d.y) >= kSlowDir && fabsf(d.z) >= kSlowDir);
+    constexpr int M1 = CULL ? 1 : 0, M2 = CULL ? 2 : 0;
+    if (fw.slow) {
+        map_pass<M1, false>(fw, sc, t, idx, bc);
+        map_pass<M2, false>(bw, sc, t, idx, bc);
+    } else {
+        // meshmapping_nobackfaces.comp:182-199 / meshmapping.comp:139-162:
+        // the backward pass starts from the forward pass's result
+        float t0 = t;
+        uint32_t i0 = idx;
+        V3 b0 = bc;
+        if (const int why = map_wide_pass<M1, kMapStack>(fw, sc, stk, t0, t, idx, bc)) {
+            t = t0; idx = i0; bc = b0;
+            map_pass<M1, false>(fw, sc, t, idx, bc);
+            atomicAdd(&redo[why - 1], 1u);
+        }
+        t0 = t; i0 = idx; b0 = bc;
+        if (const int why = map_wide_pass<M2, kMapStack>(bw, sc, stk, t0, t, idx, bc)) {
+            t = t0; idx = i0; bc = b0;
+            map_pass<M2, false>(bw, sc, t, idx, bc);
+            atomicAdd(&redo[why - 1], 1u);
+        }
+    }
+    coords[g] = make_float4(t, bc.x, bc.y, bc.z);
+    tidx[g] = idx;
+}
+
 // Eight waves per SIMD (57 VGPRs, no spill; the compiler's own choice of 66
 // leaves room for 7): more texels in flight hide the dependent node loads.
 // WT: the watertight node and triangle tests (node_visit_wt, map_tri_wt)

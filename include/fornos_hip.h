@@ -182,6 +182,24 @@ fornos_status fornos_meshmap_reset(fornos_meshmap *mm);
 #define FORNOS_PREDICATE_REFERENCE 0
 #define FORNOS_PREDICATE_WATERTIGHT 1
 fornos_status fornos_meshmap_set_predicate(fornos_meshmap *mm, int predicate);
+/* Tree the mapping's passes walk with the reference predicate.
+ * FORNOS_TRAVERSAL_WIDE (default): the 4-wide culling tree, nearest child
+ * first, boxes skipped once entered beyond the best hit; a hit counts only
+ * under the reference's reachability (its leaf's chain passes with the hit's
+ * t; ties go to the smaller soup index, the reference's DFS order), and a
+ * pass that cannot be decided so, or whose stack would overflow, is redone
+ * on the reference tree.  FORNOS_TRAVERSAL_REFERENCE: the reference's own
+ * stackless walk (Shaders/meshmapping_nobackfaces.comp:102-121).  Results
+ * are identical; the watertight predicate always walks the reference tree.
+ * No reference counterpart. */
+#define FORNOS_TRAVERSAL_WIDE 0
+#define FORNOS_TRAVERSAL_REFERENCE 1
+fornos_status fornos_meshmap_set_traversal(fornos_meshmap *mm, int traversal);
+/* Passes of the mapping runs since creation or the last reset that were
+ * redone on the reference tree: *undecided for a hit whose reachability the
+ * wide traversal could not decide, *overflow for a full stack.  Waits for
+ * the scene's stream. */
+fornos_status fornos_meshmap_redo_counts(fornos_meshmap *mm, uint32_t *undecided, uint32_t *overflow);
 /* Asynchronous submission of the remaining mapping work as `parts` launches
  * (1..4096) with a completion mark after each; returns without waiting.
  * Replaces the reference's one-runStep-per-frame pacing

@@ -1,0 +1,75 @@
+"""The mapping on the 4-wide culling tree (fornos_meshmap_set_traversal, the
+default) against the reference tree's own stackless walk and the oracle:
+coords and hit ids bit for bit, with and without back-face culling.  A flat
+grid forces the way back to the reference tree: a leaf box of zero height is
+entered exactly at the hit's t, so whether the reference reached the leaf
+depends on its visit order, and every such pass is redone on the reference
+tree (a full stack takes the same redo)."""
+import numpy as np
+import pytest
+
+import fornos_amd as fa
+import oracle as orc
+from bakes import MISS, Bake, same_bits
+from meshes import grid
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = {"sphere512": "sphere512",
+             "field": fa.configs.scaled("heightfield2048", size=256, high_quads=160),
+             "knot": fa.configs.scaled("knot4096", size=512, high_quads=256)}
+
+
+def _map(tex, scene, cull, traversal):
+    mm = fa.MeshMapping().init(tex, scene, cull)
+    mm.set_traversal(traversal)
+    mm.run_all()
+    mm.reset()
+    ms = mm.run_all()
+    c, t = mm.read()
+    return c, t, mm.redo_counts(), ms
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+@pytest.mark.parametrize("cull", [True, False])
+def test_wide_mapping_equals_reference_walk(gpu, name, cull):
+    b = Bake(WORKLOADS[name])
+    c0, t0, r0, ms0 = _map(b.prep["texels"], b.scene, cull, fa.MeshMapping.TRAVERSAL_REFERENCE)
+    c1, t1, r1, ms1 = _map(b.prep["texels"], b.scene, cull, fa.MeshMapping.TRAVERSAL_WIDE)
+    print("%s cull=%d: reference walk %.3f ms, wide %.3f ms, redone passes %s" % (name, cull, ms0, ms1, r1))
+    assert r0 == {"undecided": 0, "overflow": 0}
+    assert np.array_equal(t0, t1) and same_bits(c0, c1)
+    if cull:  # the Bake's own mapping (default traversal) against the oracle
+        oc, ot = b.oracle_map(cull=True)
+        assert np.array_equal(t1, ot) and same_bits(c1, oc)
+    n = len(t1)
+    assert r1["undecided"] + r1["overflow"] <= 2 * n // 100
+
+
+@pytest.mark.parametrize("cull", [True, False])
+def test_flat_leaf_boxes_redo_on_the_reference_tree(gpu, cull):
+    m = grid(32)
+    hi = fa.Mesh.from_arrays(m["positions"], m["vertices"], m["triangles"], m["normals"], m["texcoords"])
+    scene = fa.Scene(hi)
+    rng = np.random.default_rng(11)
+    n = 4096
+    # slopes up to 0.3 over a rise of 0.3: every ray meets the grid inside [0, 1]^2
+    p = np.column_stack([rng.uniform(0.1, 0.9, n), rng.uniform(0.1, 0.9, n),
+                         np.full(n, -0.3)]).astype(np.float32)
+    d = np.column_stack([rng.uniform(-0.3, 0.3, n), rng.uniform(-0.3, 0.3, n), np.ones(n)])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    assert np.all(np.abs(d) >= 2.0 ** -40)  # none of them takes the reference tree from the start
+    z = np.zeros((n, 3), np.float32)
+    tex = fa.TexelArrays({"positions": p, "directions": d, "normals": z, "tangents": z, "bitangents": z,
+                          "indices": np.arange(n, dtype=np.uint32)}, n, 1)
+    c1, t1, r1, _ = _map(tex, scene, cull, fa.MeshMapping.TRAVERSAL_WIDE)
+    c0, t0, _, _ = _map(tex, scene, cull, fa.MeshMapping.TRAVERSAL_REFERENCE)
+    nodes, spos, _, _ = orc.bvh_build(m)
+    oc, ot = orc.meshmapping(p, d, spos, nodes, cull)
+    print("flat grid cull=%d: redone passes %s" % (cull, r1))
+    assert np.all(ot != MISS)
+    assert np.array_equal(t1, ot) and same_bits(c1, oc)
+    assert np.array_equal(t0, ot) and same_bits(c0, oc)
+    assert r1["undecided"] > 0
+    with pytest.raises(fa.FornosError):
+        fa.MeshMapping().init(tex, scene, cull).set_traversal(2)
