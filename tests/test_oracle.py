@@ -208,6 +208,12 @@ def test_bvh_invariants_large():
         if parent[i] >= 0:
             p = parent[i]
             assert np.all(lo[p] <= lo[i] + 1e-6) and np.all(hi[p] >= hi[i] - 1e-6)
+    # the soup holds the leaves' triangles in the order the stackless DFS
+    # visits the leaves (node order): a smaller soup index is visited first,
+    # which is how the wide mapping breaks ties in t
+    leaves = [(nodes[i, 6], nodes[i, 7]) for i in range(len(nodes)) if nodes[i, 7] > nodes[i, 6]]
+    assert leaves[0][0] == 0 and leaves[-1][1] == 3 * len(m["triangles"])
+    assert all(a1 == b0 for (_, a1), (b0, _) in zip(leaves, leaves[1:]))
 
 
 # ---------------------------------------------------------------- mapping --
