@@ -55,7 +55,7 @@ def load_profile(workload):
         return json.load(f)
 
 
-def ao_pass(fa, workload, steps, warmup, log):
+def ao_pass(fa, workload, steps, warmup, log, tune=True, launch=None):
     """Prepares a workload and times its AO pass: returns the figures and the
     live objects (prep, mapping, solver)."""
     import numpy as np
@@ -75,6 +75,19 @@ def ao_pass(fa, workload, steps, warmup, log):
     _, tidx = mm.read()
     live = int(np.sum(tidx != 0xFFFFFFFF))
     S = p["sampleCount"]
+    tuning = None
+    if launch:
+        ao.set_launch(*launch)
+        tuning = {"blocks_per_cu": launch[0], "refill": launch[1], "stack": launch[2], "tuned": False}
+    elif tune:
+        # launch shape timed on 5 % of the pass (results never depend on it);
+        # its cost is set-up, outside the timed steps, and reported
+        t0 = time.perf_counter()
+        bpc, refill, stack, tms = ao.autotune(0.05)
+        tuning = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "tuned": True,
+                  "seconds": time.perf_counter() - t0, "fraction": 0.05}
+        log("# %s autotuned launch shape: %d blocks/CU, refill at %d idle lanes, stack %d (%.2f s)"
+            % (workload, bpc, refill, stack, tuning["seconds"]))
     for _ in range(warmup):
         ao.reset()
         ao.run_all()
@@ -92,7 +105,8 @@ def ao_pass(fa, workload, steps, warmup, log):
            "wall": wall, "steps": steps, "ms_per_step": 1000.0 * wall / steps,
            "value": live * S * steps / wall, "launches": launches,
            "launch_ms": float(np.mean(rays_ms)) / launches, "reduce_ms": float(np.mean(reduce_ms)),
-           "pass_kernel_ms": float(np.mean(pass_ms)),
+           "pass_kernel_ms": float(np.mean(pass_ms)), "launch_shape": tuning or {"blocks_per_cu": 8, "refill": 48,
+                                                                                 "stack": 16, "tuned": False},
            "ray_order": ("Morton code of the texels' hit positions"
                          if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel set order")}
     log("# %s AO-%d: %.2f ms/step, %.3e rays/s" % (workload, S, res["ms_per_step"], res["value"]))
@@ -113,6 +127,12 @@ def roofline(res):
            "l2_model_bytes_per_launch": l2_bytes, "l2_model_gbs": l2_bytes / t / 1e9}
     prof = load_profile(res["workload"]) or {}
     hbm, valu = prof.get("hbm_bytes_per_launch"), prof.get("valu_insts_per_launch")
+    shape = res["launch_shape"]
+    pshape = prof.get("launch_shape")
+    out["profile_launch_shape"] = pshape
+    if pshape and [shape["blocks_per_cu"], shape["refill"], shape["stack"]] != list(pshape):
+        out["profile_note"] = ("counters taken at launch shape %s, this run used %s"
+                               % (pshape, [shape["blocks_per_cu"], shape["refill"], shape["stack"]]))
     cands = []
     if valu:
         tf = valu * FLOPS_PER_VALU_INST / t / 1e12
@@ -249,6 +269,10 @@ def main():
     ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--no-knot", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
+    ap.add_argument("--launch", default=None,
+                    help="BLOCKS_PER_CU,REFILL,STACK: this launch shape for the headline workload instead of tuning "
+                         "(the profiles are taken at the tuned shape this way)")
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy, the known-size kernel that scales the "
@@ -279,12 +303,13 @@ def main():
         hip.hipFree(a)
         hip.hipFree(b)
 
-    head = ao_pass(fa, args.workload, args.steps, args.warmup, log)
+    launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
+    head = ao_pass(fa, args.workload, args.steps, args.warmup, log, tune=not args.no_tune, launch=launch)
     roof = roofline(head)
 
     workloads = {}
     if not args.no_knot and args.workload != "knot4096":
-        k = ao_pass(fa, "knot4096", 1, 1, log)
+        k = ao_pass(fa, "knot4096", 1, 1, log, tune=not args.no_tune)
         workloads["knot4096"] = {
             "metric": "AO rays/s", "value": k["value"], "unit": "rays/s", "ms_per_step": k["ms_per_step"],
             "steps": 1, "warmup": 1,
@@ -292,7 +317,7 @@ def main():
                       % (k["cfg"]["size"], k["info"]["n_triangles"], k["S"], k["params"]["minDistance"],
                          k["params"]["maxDistance"]),
             "texels": k["texels"], "rays_per_step": k["rays_per_step"], "mapping_ms": k["map_ms"],
-            "ray_order": k["ray_order"],
+            "ray_order": k["ray_order"], "launch_shape": k["launch_shape"],
             "roofline": roofline(k)}
         del k
 
@@ -326,7 +351,7 @@ def main():
                                   p["maxDistance"]),
                    "texels": head["texels"], "rays_per_step": head["rays_per_step"],
                    "bvh": "reference binary tree (exact reachability) + 4-wide SAH culling tree",
-                   "ray_order": head["ray_order"]},
+                   "ray_order": head["ray_order"], "launch_shape": head["launch_shape"]},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity, "mapping_ms": head["map_ms"],
         "texels_per_s": head["live"] / (head["ms_per_step"] * 1e-3),
         "workloads": workloads, "batch": batch, "bake": bake, "denoise": denoise,

@@ -423,6 +423,14 @@ class Solver(_Handle):
         check(lib().fornos_solver_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def autotune(self, fraction=0.05):
+        """Times launch shapes on the first `fraction` of the pass's texels
+        and keeps the fastest; returns (blocks_per_cu, refill, stack, ms)
+        (fornos_solver_autotune).  Results never depend on the shape."""
+        a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().fornos_solver_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(c), C.byref(ms)))
+        return a.value, b.value, c.value, ms.value
+
     RAY_ORDER_TEXELS, RAY_ORDER_MORTON, RAY_ORDER_AUTO = 0, 1, 2
 
     def set_ray_order(self, order):

@@ -1458,6 +1458,67 @@ fornos_status fornos_solver_set_launch(fornos_solver* s, uint32_t blocks_per_cu,
     return FORNOS_OK;
 }
 
+fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t* best_bpc, uint32_t* best_refill,
+                                     uint32_t* best_stack, float* best_ms)
+{
+    if (!s || !(fraction > 0.0f) || fraction > 1.0f)
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_autotune: fraction must be in (0, 1]");
+    if (!s->hemi())
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_autotune: not a hemisphere solver (AO, bent normals, thickness)");
+    if (s->owner)
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_autotune: a paired bent-normals solver is tuned through its AO solver");
+    const uint32_t n = s->mm->n, S = s->p.sample_count;
+    // the first `fraction` of the texels in the pass's own order, at least
+    // 2^22 rays, so the launch reaches its steady state
+    const uint32_t m = std::min<uint32_t>(n, std::max<uint32_t>(div_up(1u << 22, S), (uint32_t)((double)n * fraction)));
+    const uint32_t* list = nullptr;
+    if (fornos_status e = pass_order(s, &list)) return e;
+    hipStream_t st = s->mm->sc->stream;
+    const uint32_t keep[3] = {s->blocks_per_cu, s->refill, s->stack};
+    uint32_t cur[3] = {keep[0], keep[1], keep[2]};  // blocks per CU, refill, stack
+    auto time_shape = [&](const uint32_t* shape, float* ms) -> fornos_status {
+        s->blocks_per_cu = shape[0];
+        s->refill = shape[1];
+        s->stack = shape[2];
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
+            Timer tm(st);
+            if (fornos_status e = tm.start()) return e;
+            const fornos_status e = s->partner ? launch_pair(s, 0, m, list) : solver_launch(s, 0, m, false, list);
+            if (e) return e;
+            if (fornos_status e2 = tm.stop(ms)) return e2;
+        }
+        return FORNOS_OK;
+    };
+    // coordinate descent: stack depth, then blocks per CU, then refill threshold
+    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {32, 40, 48, 52, 56, 60}, {16, 24, 32}};
+    float best = FLT_MAX;
+    fornos_status err = FORNOS_OK;
+    for (int axis : {2, 0, 1}) {
+        uint32_t pick = cur[axis];
+        for (uint32_t v : cand[axis]) {
+            uint32_t shape[3] = {cur[0], cur[1], cur[2]};
+            shape[axis] = v;
+            float ms = 0;
+            if ((err = time_shape(shape, &ms))) break;
+            if (ms < best) {
+                best = ms;
+                pick = v;
+            }
+        }
+        if (err) break;
+        cur[axis] = pick;
+    }
+    s->blocks_per_cu = err ? keep[0] : cur[0];
+    s->refill = err ? keep[1] : cur[1];
+    s->stack = err ? keep[2] : cur[2];
+    if (err) return err;
+    if (best_bpc) *best_bpc = cur[0];
+    if (best_refill) *best_refill = cur[1];
+    if (best_stack) *best_stack = cur[2];
+    if (best_ms) *best_ms = best;
+    return FORNOS_OK;
+}
+
 fornos_status fornos_solver_set_ray_order(fornos_solver* s, int order)
 {
     if (!s || (order != FORNOS_RAY_ORDER_TEXELS && order != FORNOS_RAY_ORDER_MORTON && order != FORNOS_RAY_ORDER_AUTO))

@@ -347,6 +347,17 @@ fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu,
                                        uint32_t stack);
 fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
                                        uint32_t *stack);
+/* Autotuner of a hemisphere solver's launch shape: times its pass over the
+ * first `fraction` of its texels in the pass's order (at least 2^22 rays;
+ * the paired AO + bent-normals traversal when paired) by coordinate descent
+ * -- stack depth {16, 24, 32}, then blocks per CU {4, 6, 8, 12}, then the
+ * refill threshold {32, 40, 48, 52, 56, 60} -- and keeps the fastest (the
+ * best refill threshold differs by scene: heightfield2048 AO 56, knot4096
+ * AO 40).  Results never depend on the shape; the sampled texels are left
+ * with their pass results and the solver's progress is unchanged.  No
+ * reference counterpart. */
+fornos_status fornos_solver_autotune(fornos_solver *s, float fraction, uint32_t *blocks_per_cu, uint32_t *refill,
+                                     uint32_t *stack, float *best_ms);
 /* Order in which a hemisphere solver's passes hand out their texels' rays.
  * FORNOS_RAY_ORDER_MORTON: texels sorted by the Morton code of their hit
  * positions (built on the device once per mapping result), so the rays in

@@ -279,3 +279,38 @@ def test_ao_tree_of_split_leaves(field):
     b.run_all()
     wb = b.count_work()
     assert w["tri_tests"] < wb["tri_tests"]  # the groups cut AO's triangle tests
+
+
+@pytest.mark.parametrize("kind", ["ao", "thickness", "paired"])
+def test_autotune_picks_a_shape_and_keeps_results(field, kind):
+    # fornos_solver_autotune times launch shapes on a prefix of the pass;
+    # the shape it keeps changes nothing in the results
+    S, tex = 32, field.prep["texels"]
+    if kind == "thickness":
+        make = lambda: fa.ThicknessSolver(S, 0.01, 0.5).init(tex, field.mm)
+    else:
+        make = lambda: fa.AmbientOcclusionSolver(S, 0.01, 0.2).init(tex, field.mm)
+    ref = make()
+    ref_bn = fa.BentNormalsSolver(S, 0.01, 0.2, True).init(tex, field.mm) if kind == "paired" else None
+    ref.run_all()
+    if ref_bn:
+        ref_bn.run_all()
+    s = make()
+    bn = None
+    if kind == "paired":
+        bn = fa.BentNormalsSolver(S, 0.01, 0.2, True).init(tex, field.mm)
+        s.pair(bn)
+        with pytest.raises(fa.FornosError):
+            bn.autotune(0.1)  # tuned through its AO solver
+    bpc, refill, stack, ms = s.autotune(0.1)
+    assert s.get_launch() == (bpc, refill, stack)
+    assert bpc in (4, 6, 8, 12) and refill in (32, 40, 48, 52, 56, 60) and stack in (16, 24, 32) and ms > 0
+    s.reset()
+    s.run_all()
+    assert same_bits(s.getResults(), ref.getResults())
+    if bn:
+        assert bn.run_all() >= 0 and same_bits(bn.getResults(), ref_bn.getResults())
+    with pytest.raises(fa.FornosError):
+        s.autotune(0.0)
+    with pytest.raises(fa.FornosError):
+        fa.HeightSolver(True, 0.0).init(tex, field.mm).autotune(0.1)
