@@ -49,7 +49,9 @@ void launch_conv(const ConvArgs& A, hipStream_t st)
         hipLaunchKernelGGL((k_conv3x3<MT, 2>), dim3(tiles, A.cout / 32), dim3(kThreads), 0, st, A, tx, ty);
 }
 
-void launch_wgrad(WgradArgs A, hipStream_t st)
+// k_wgrad + k_wgrad_reduce; `part` is their partial-tile buffer (grown when
+// a layer needs more, after the stream's earlier work).
+fornos_status launch_wgrad(WgradArgs A, hipStream_t st, float*& part, size_t& part_cap)
 {
     const uint64_t P = (uint64_t)A.n * A.h * A.w;
     const uint32_t cin = A.c1 + A.c2;
@@ -59,8 +61,22 @@ void launch_wgrad(WgradArgs A, hipStream_t st)
     const uint32_t chunks = div_up(P, kWChunk);
     const uint32_t splits = std::max<uint32_t>(1, std::min<uint32_t>(chunks, div_up(4096, tiles)));
     A.chunk_per_block = div_up(chunks, splits);
-    const dim3 grid(div_up(chunks, A.chunk_per_block), tiles);
-    hipLaunchKernelGGL(k_wgrad, grid, dim3(kThreads), 0, st, A);
+    A.splits = div_up(chunks, A.chunk_per_block);
+    const size_t need = (size_t)A.splits * (tiles * (size_t)kWTile + A.co_tiles * (size_t)64);
+    if (need > part_cap) {
+        HIP_TRY(hipStreamSynchronize(st));
+        dev_free(part);
+        part = nullptr;
+        part_cap = 0;
+        if (fornos_status e = dev_alloc(&part, need)) return e;
+        part_cap = need;
+    }
+    A.part = part;
+    hipLaunchKernelGGL(k_wgrad, dim3(A.splits, tiles), dim3(kThreads), 0, st, A);
+    const size_t nred = (size_t)A.cout * 9 * cin + (A.db ? A.cout : 0);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(div_up(nred, kThreads)), dim3(kThreads), 0, st, A);
+    HIP_TRY(hipGetLastError());
+    return FORNOS_OK;
 }
 
 template <typename K, typename... Args>
@@ -90,12 +106,14 @@ struct fornos_denoiser {
     float* out = nullptr;
     uint32_t* valid = nullptr;
     float* loss = nullptr;
+    float* wpart = nullptr;  // weight-gradient partial tiles (launch_wgrad)
+    size_t wpart_cap = 0;
     hipStream_t st = nullptr;  // the default stream
 
     ~fornos_denoiser()
     {
         (void)hipDeviceSynchronize();
-        for (void* p : {(void*)params, (void*)grads, (void*)m, (void*)v, (void*)wb, (void*)wflip}) dev_free(p);
+        for (void* p : {(void*)params, (void*)grads, (void*)m, (void*)v, (void*)wb, (void*)wflip, (void*)wpart}) dev_free(p);
         free_acts();
     }
     void free_acts()
@@ -235,7 +253,7 @@ struct fornos_denoiser {
     }
 
     // weight gradient of layer i, then the gradient of its inputs
-    void conv_backward(uint32_t i, uint32_t n, uint32_t h, uint32_t w)
+    fornos_status conv_backward(uint32_t i, uint32_t n, uint32_t h, uint32_t w)
     {
         const Layer& L = layers[i];
         const uint32_t hl = h >> L.level, wl = w >> L.level;
@@ -251,8 +269,8 @@ struct fornos_denoiser {
         W.h = hl;
         W.w = wl;
         W.up_a = (int)L.up;
-        launch_wgrad(W, st);
-        if (i == 0) return;  // no gradient for the network input
+        if (fornos_status e = launch_wgrad(W, st, wpart, wpart_cap)) return e;
+        if (i == 0) return FORNOS_OK;  // no gradient for the network input
 
         ConvArgs A{};
         A.a = gact[i];
@@ -293,6 +311,7 @@ struct fornos_denoiser {
                           (const uint16_t*)act[bl], gact[bl], n, hl >> 1, wl >> 1, ch(l + 1));
             }
         }
+        return FORNOS_OK;
     }
 
     void refresh_weights()
@@ -455,12 +474,12 @@ fornos_status fornos_denoiser_backward(fornos_denoiser* d, const float* input, c
                        (const float*)(d->params + d->head_w), inv_valid, d->gact[d->dec(0, 1)],
                        d->grads + d->head_w, d->grads + d->head_b, d->loss, (size_t)P);
     for (uint32_t l = 0; l + 1 < d->levels; ++l) {
-        d->conv_backward(d->dec(l, 1), n, h, w);
-        d->conv_backward(d->dec(l, 0), n, h, w);
+        if (fornos_status e = d->conv_backward(d->dec(l, 1), n, h, w)) return e;
+        if (fornos_status e = d->conv_backward(d->dec(l, 0), n, h, w)) return e;
     }
     for (int l = (int)d->levels - 1; l >= 0; --l) {
-        d->conv_backward(d->enc(l, 1), n, h, w);
-        d->conv_backward(d->enc(l, 0), n, h, w);
+        if (fornos_status e = d->conv_backward(d->enc(l, 1), n, h, w)) return e;
+        if (fornos_status e = d->conv_backward(d->enc(l, 0), n, h, w)) return e;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
@@ -550,10 +569,12 @@ fornos_status fornos_dn_conv3x3_wgrad(const uint16_t* a, uint32_t c1, int up_a, 
     W.h = h;
     W.w = w;
     W.up_a = up_a ? 1 : 0;
-    launch_wgrad(W, nullptr);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipDeviceSynchronize());
-    return FORNOS_OK;
+    float* part = nullptr;
+    size_t cap = 0;
+    fornos_status e = launch_wgrad(W, nullptr, part, cap);
+    if (!e && hipDeviceSynchronize() != hipSuccess) e = set_error(FORNOS_HIP_ERROR, "fornos_dn_conv3x3_wgrad: device error");
+    dev_free(part);
+    return e;
 }
 
 }  // extern "C"

@@ -242,23 +242,28 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
 //   dw[co][t][ci] += sum_p g[p][co] * x_t[p][ci],  db[co] += sum_p g[p][co]
 // x_t is the layer input shifted by tap t, read with the same concat /
 // upsample rules as the forward pass.  A block owns a 64 x 64 (co, ci) tile
-// of one tap over a range of pixels; 32-pixel chunks of g and x are staged in
-// LDS transposed (channel-major) so each lane reads its 8 consecutive pixels
-// as one 16-byte operand (K = pixels).  Partial tiles are added with fp32
-// atomics into dw, which the caller zeroes.
+// of one tap over a range of pixels (one split of K = pixels); 32-pixel
+// chunks of g and x are staged in LDS transposed (channel-major) so each
+// lane reads its 8 consecutive pixels as one 16-byte operand, and the next
+// chunk's loads are in flight while the current chunk's MFMAs run.  Each
+// block stores its partial tile (and, for the centre tap of the first input
+// tile, its partial bias) in `part`; k_wgrad_reduce adds the splits in order,
+// so the result does not depend on the order blocks finish in.
 struct WgradArgs {
     const uint16_t* a;
     const uint16_t* b;
     const uint16_t* g;  // [pixels][cout]
     float* dw;          // [cout][9][c1 + c2]
     float* db;          // [cout] or null
+    float* part;        // [tiles][splits][64 x 64] partial tiles, then [co_tiles][splits][64] partial biases
     uint32_t c1, c2, cout, n, h, w;
     int up_a;
-    uint32_t ci_tiles, co_tiles, chunk_per_block;  // chunks of 32 pixels per block
+    uint32_t ci_tiles, co_tiles, chunk_per_block, splits;  // chunks of 32 pixels per block, blocks per tile
 };
 
 constexpr int kWChunk = 32;
 constexpr int kWPad = kWChunk + 8;
+constexpr int kWTile = 64 * 64;
 
 __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs A)
 {
@@ -291,25 +296,29 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs A)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
     float bsum = 0.0f;
 
-    for (uint32_t p0 = p_begin; p0 < p_end; p0 += kWChunk) {
+    auto load = [&](uint32_t p0, uint4& gv, uint4& xv) {
         const uint32_t p = p0 + lp;
-        uint4 gv = make_uint4(0, 0, 0, 0), xv = make_uint4(0, 0, 0, 0);
-        if (p < p_end) {
-            const uint32_t co = co0 + lc;
-            if (co < A.cout) gv = *reinterpret_cast<const uint4*>(A.g + (size_t)p * A.cout + co);
-            const uint32_t ci = ci0 + lc;
-            const uint32_t pn = p / hw, rem = p - pn * hw;
-            const int y = (int)(rem / A.w) + dy, x = (int)(rem % A.w) + dx;
-            if (ci < cin && y >= 0 && y < (int)A.h && x >= 0 && x < (int)A.w) {
-                const uint16_t* src;
-                if (ci < A.c1)
-                    src = A.up_a ? A.a + (((size_t)pn * hh + (y >> 1)) * wh + (x >> 1)) * A.c1 + ci
-                                 : A.a + (((size_t)pn * A.h + y) * A.w + x) * A.c1 + ci;
-                else
-                    src = A.b + (((size_t)pn * A.h + y) * A.w + x) * A.c2 + (ci - A.c1);
-                xv = *reinterpret_cast<const uint4*>(src);
-            }
+        gv = make_uint4(0, 0, 0, 0);
+        xv = make_uint4(0, 0, 0, 0);
+        if (p >= p_end) return;
+        const uint32_t co = co0 + lc;
+        if (co < A.cout) gv = *reinterpret_cast<const uint4*>(A.g + (size_t)p * A.cout + co);
+        const uint32_t ci = ci0 + lc;
+        const uint32_t pn = p / hw, rem = p - pn * hw;
+        const int y = (int)(rem / A.w) + dy, x = (int)(rem % A.w) + dx;
+        if (ci < cin && y >= 0 && y < (int)A.h && x >= 0 && x < (int)A.w) {
+            const uint16_t* src;
+            if (ci < A.c1)
+                src = A.up_a ? A.a + (((size_t)pn * hh + (y >> 1)) * wh + (x >> 1)) * A.c1 + ci
+                             : A.a + (((size_t)pn * A.h + y) * A.w + x) * A.c1 + ci;
+            else
+                src = A.b + (((size_t)pn * A.h + y) * A.w + x) * A.c2 + (ci - A.c1);
+            xv = *reinterpret_cast<const uint4*>(src);
         }
+    };
+    uint4 gv, xv;
+    load(p_begin, gv, xv);
+    for (uint32_t p0 = p_begin; p0 < p_end; p0 += kWChunk) {
         __syncthreads();  // the previous chunk's reads are done
         const uint16_t* g8 = reinterpret_cast<const uint16_t*>(&gv);
         const uint16_t* x8 = reinterpret_cast<const uint16_t*>(&xv);
@@ -318,6 +327,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs A)
             sg[(lc + j) * kWPad + lp] = g8[j];
             sx[(lc + j) * kWPad + lp] = x8[j];
         }
+        if (p0 + kWChunk < p_end) load(p0 + kWChunk, gv, xv);
         __syncthreads();
         if (do_bias && tid < 64) {
             for (int q = 0; q < kWChunk; ++q) bsum += bf2f(sg[tid * kWPad + q]);
@@ -333,17 +343,43 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs A)
             for (int j = 0; j < 2; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bx[j], acc[i][j], 0, 0, 0);
     }
+    float* tile = A.part + ((size_t)blockIdx.y * A.splits + blockIdx.x) * kWTile;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t co = co0 + wco + i * 16 + kq * 4 + e;
-                const uint32_t ci = ci0 + wci + j * 16 + r;
-                if (co < A.cout && ci < cin) atomicAdd(&A.dw[((size_t)co * 9 + t) * cin + ci], acc[i][j][e]);
-            }
-    if (do_bias && tid < 64 && co0 + tid < A.cout) atomicAdd(&A.db[co0 + tid], bsum);
+            for (int e = 0; e < 4; ++e)
+                tile[(wco + i * 16 + kq * 4 + e) * 64 + wci + j * 16 + r] = acc[i][j][e];
+    if (do_bias && tid < 64) {
+        float* bp = A.part + (size_t)gridDim.y * A.splits * kWTile;
+        bp[((size_t)ot * A.splits + blockIdx.x) * 64 + tid] = bsum;
+    }
+}
+
+// Second pass of k_wgrad: dw[co][t][ci] += the splits of its tile in split
+// order (and db[co] likewise), one thread per weight.
+__global__ void k_wgrad_reduce(WgradArgs A)
+{
+    const uint32_t cin = A.c1 + A.c2;
+    const size_t nw = (size_t)A.cout * 9 * cin;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw) {
+        const uint32_t ci = (uint32_t)(i % cin), rest = (uint32_t)(i / cin);
+        const uint32_t t = rest % 9, co = rest / 9;
+        const uint32_t tile = (t * A.co_tiles + co / 64) * A.ci_tiles + ci / 64;
+        const float* p = A.part + (size_t)tile * A.splits * kWTile + (co % 64) * 64 + ci % 64;
+        float s = 0.0f;
+        for (uint32_t k = 0; k < A.splits; ++k) s += p[(size_t)k * kWTile];
+        A.dw[i] += s;
+    } else if (A.db && i < nw + A.cout) {
+        const uint32_t co = (uint32_t)(i - nw);
+        const float* p = A.part + (size_t)9 * A.co_tiles * A.ci_tiles * A.splits * kWTile +
+                         (size_t)(co / 64) * A.splits * 64 + co % 64;
+        float s = 0.0f;
+        for (uint32_t k = 0; k < A.splits; ++k) s += p[(size_t)k * 64];
+        A.db[co] += s;
+    }
 }
 
 // 2x2 average pooling, 8 channels per thread: out[n][y][x] = mean of the

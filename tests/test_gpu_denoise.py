@@ -54,7 +54,8 @@ def test_conv3x3_matches_oracle(torch, up, c1, c2, cout, relu, hw):
 
 
 @pytest.mark.parametrize("up,c1,c2,cout,hw", [(0, 32, 0, 32, (16, 16)), (1, 64, 32, 32, (8, 12)),
-                                              (0, 128, 0, 64, (32, 32)), (1, 256, 128, 128, (4, 4))])
+                                              (0, 128, 0, 64, (32, 32)), (1, 256, 128, 128, (4, 4)),
+                                              (0, 32, 0, 32, (96, 128))])
 def test_wgrad_matches_oracle(torch, up, c1, c2, cout, hw):
     from fornos_amd._abi import check, lib
     rng = np.random.default_rng(c1 + c2 + cout)
@@ -73,6 +74,12 @@ def test_wgrad_matches_oracle(torch, up, c1, c2, cout, hw):
                                np.abs(f(g)), cout)
     assert np.all(np.abs(dw.cpu().numpy() - rw) <= 1e-5 * mw + 1e-6)
     assert np.all(np.abs(db.cpu().numpy() - rb) <= 1e-5 * mb + 1e-6)
+    # the splits of K are added in a fixed order: a second run gives the same bits
+    dw2, db2 = torch.empty_like(dw), torch.empty_like(db)
+    check(lib().fornos_dn_conv3x3_wgrad(p(a), c1, up, p(b) if b is not None else None, c2, n, h, w, p(g), cout,
+                                        p(dw2), p(db2)))
+    assert torch.equal(dw.view(torch.int32), dw2.view(torch.int32))
+    assert torch.equal(db.view(torch.int32), db2.view(torch.int32))
 
 
 def small_batch(n, h, w, seed):
