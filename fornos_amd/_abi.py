@@ -124,7 +124,7 @@ _SIGS = [
     ("fornos_meshmap_progress", None, [H, C.POINTER(C.c_float)]),
     ("fornos_meshmap_reset", None, [H]),
     ("fornos_meshmap_set_predicate", None, [H, C.c_int]),
-    ("fornos_meshmap_set_traversal", None, [H, C.c_int]),
+    ("fornos_meshmap_set_traversal", None, [H, C.c_int, C.c_uint32]),
     ("fornos_meshmap_redo_counts", None, [H, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("fornos_meshmap_submit", None, [H, C.c_uint32]),
     ("fornos_meshmap_poll", None, [H, C.POINTER(C.c_float), C.POINTER(C.c_int)]),

@@ -285,11 +285,12 @@ class MeshMapping(_Handle):
 
     TRAVERSAL_WIDE, TRAVERSAL_REFERENCE = 0, 1
 
-    def set_traversal(self, traversal):
+    def set_traversal(self, traversal, stack=0):
         """TRAVERSAL_WIDE (default) or TRAVERSAL_REFERENCE: the tree the
-        reference predicate's passes walk; results are identical
+        reference predicate's passes walk; `stack`: entries of the wide
+        traversal's stack (3..24, 0 = 24).  Results are identical
         (fornos_meshmap_set_traversal)."""
-        check(lib().fornos_meshmap_set_traversal(self._h, int(traversal)))
+        check(lib().fornos_meshmap_set_traversal(self._h, int(traversal), int(stack)))
 
     def redo_counts(self):
         """{"undecided": passes redone on the reference tree for a hit whose

@@ -105,6 +105,7 @@ struct fornos_meshmap {
     int cull = 1;
     int predicate = FORNOS_PREDICATE_REFERENCE;  // triangle test of the mapping
     int traversal = FORNOS_TRAVERSAL_WIDE;        // tree the reference predicate's passes walk
+    uint32_t wide_stack = kMapStack;              // stack entries the wide passes may use
     uint32_t* redo = nullptr;  // passes redone on the reference tree: undecidable hit, stack overflow
     Marks marks;              // parts of the last fornos_meshmap_submit
     uint32_t mark_from = 0;   // texel its first part started at
@@ -912,10 +913,10 @@ static fornos_status meshmap_launch(fornos_meshmap* mm, uint32_t t0, uint32_t t1
     if (!wt && mm->traversal == FORNOS_TRAVERSAL_WIDE && d.n_cnodes > 0) {
         if (mm->cull)
             hipLaunchKernelGGL((k_meshmap_wide<true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords,
-                               mm->tidx, mm->redo);
+                               mm->tidx, mm->redo, mm->wide_stack);
         else
             hipLaunchKernelGGL((k_meshmap_wide<false>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords,
-                               mm->tidx, mm->redo);
+                               mm->tidx, mm->redo, mm->wide_stack);
     } else if (mm->cull && wt) {
         hipLaunchKernelGGL((k_meshmap<true, true>), dim3(blocks), dim3(kBlock), 0, st, mm->pix, t0, t1, d, mm->coords, mm->tidx);
     } else if (mm->cull) {
@@ -972,11 +973,14 @@ fornos_status fornos_meshmap_reset(fornos_meshmap* mm)
     return FORNOS_OK;
 }
 
-fornos_status fornos_meshmap_set_traversal(fornos_meshmap* mm, int traversal)
+fornos_status fornos_meshmap_set_traversal(fornos_meshmap* mm, int traversal, uint32_t stack)
 {
     if (!mm || (traversal != FORNOS_TRAVERSAL_WIDE && traversal != FORNOS_TRAVERSAL_REFERENCE))
         return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_set_traversal: FORNOS_TRAVERSAL_WIDE or _REFERENCE");
+    if (stack != 0 && (stack < 3 || stack > (uint32_t)kMapStack))
+        return set_error(FORNOS_INVALID_ARG, "fornos_meshmap_set_traversal: stack 0 (default) or 3..24 entries");
     mm->traversal = traversal;
+    mm->wide_stack = stack ? stack : (uint32_t)kMapStack;
     return FORNOS_OK;
 }
 

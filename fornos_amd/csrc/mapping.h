@@ -156,9 +156,9 @@ __device__ inline void map_pass(const Ray& r, const DevScene& sc, float& cur, ui
 // Returns 0 when the pass is decided, kMapRedoChain + 1 or kMapRedoStack + 1
 // to redo it (cur, idx and bc are then partial; the caller restores them).
 enum { kMapRedoChain = 0, kMapRedoStack = 1 };  // counters of passes redone on the reference tree
-template <int MODE, int STK>
-__device__ inline int map_wide_pass(const Ray& r, const DevScene& sc, uint32_t* stk, float cur0, float& cur,
-                                    uint32_t& idx, V3& bc)
+template <int MODE>
+__device__ inline int map_wide_pass(const Ray& r, const DevScene& sc, uint32_t* stk, uint32_t depth, float cur0,
+                                    float& cur, uint32_t& idx, V3& bc)
 {
     const uint32_t tid = threadIdx.x;
     bool mine = false;  // the best hit is this pass's own
@@ -217,7 +217,7 @@ __device__ inline int map_wide_pass(const Ray& r, const DevScene& sc, uint32_t* 
 #undef FORNOS_CSWAP
         node = c[0];
         const uint32_t n = (uint32_t)(c[1] != kNoChild) + (uint32_t)(c[2] != kNoChild) + (uint32_t)(c[3] != kNoChild);
-        if (sp + n > STK) return kMapRedoStack + 1;
+        if (sp + n > depth) return kMapRedoStack + 1;
 #pragma unroll
         for (uint32_t k = 1; k <= 3; ++k)
             if (k <= n) stk[(sp + n - k) * kBlock + tid] = c[k];
@@ -225,7 +225,7 @@ __device__ inline int map_wide_pass(const Ray& r, const DevScene& sc, uint32_t* 
     }
 }
 
-constexpr int kMapStack = 24;  // 4-wide mapping stack entries per lane (LDS)
+constexpr int kMapStack = 24;  // 4-wide mapping stack entries per lane (LDS); `depth` may use fewer
 
 // The reference predicate on the 4-wide culling tree (map_wide_pass), each
 // pass redone on the reference tree when it cannot be decided there, and rays
@@ -234,7 +234,8 @@ constexpr int kMapStack = 24;  // 4-wide mapping stack entries per lane (LDS)
 template <bool CULL>
 __global__ __launch_bounds__(kBlock) void k_meshmap_wide(const float4* __restrict__ pix, uint32_t begin, uint32_t end,
                                                         DevScene sc, float4* __restrict__ coords,
-                                                        uint32_t* __restrict__ tidx, uint32_t* __restrict__ redo)
+                                                        uint32_t* __restrict__ tidx, uint32_t* __restrict__ redo,
+                                                        uint32_t depth)
 {
     __shared__ uint32_t stk[kMapStack * kBlock];
     const uint32_t g = begin + blockIdx.x * kBlock + threadIdx.x;
@@ -262,13 +263,13 @@ __global__ __launch_bounds__(kBlock) void k_meshmap_wide(const float4* __restric
         float t0 = t;
         uint32_t i0 = idx;
         V3 b0 = bc;
-        if (const int why = map_wide_pass<M1, kMapStack>(fw, sc, stk, t0, t, idx, bc)) {
+        if (const int why = map_wide_pass<M1>(fw, sc, stk, depth, t0, t, idx, bc)) {
             t = t0; idx = i0; bc = b0;
             map_pass<M1, false>(fw, sc, t, idx, bc);
             atomicAdd(&redo[why - 1], 1u);
         }
         t0 = t; i0 = idx; b0 = bc;
-        if (const int why = map_wide_pass<M2, kMapStack>(bw, sc, stk, t0, t, idx, bc)) {
+        if (const int why = map_wide_pass<M2>(bw, sc, stk, depth, t0, t, idx, bc)) {
             t = t0; idx = i0; bc = b0;
             map_pass<M2, false>(bw, sc, t, idx, bc);
             atomicAdd(&redo[why - 1], 1u);

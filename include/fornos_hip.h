@@ -194,7 +194,10 @@ fornos_status fornos_meshmap_set_predicate(fornos_meshmap *mm, int predicate);
  * No reference counterpart. */
 #define FORNOS_TRAVERSAL_WIDE 0
 #define FORNOS_TRAVERSAL_REFERENCE 1
-fornos_status fornos_meshmap_set_traversal(fornos_meshmap *mm, int traversal);
+/* stack: entries of the wide traversal's per-texel stack, 3..24, or 0 for
+ * the default 24; a pass that would need more is redone on the reference
+ * tree (results unchanged). */
+fornos_status fornos_meshmap_set_traversal(fornos_meshmap *mm, int traversal, uint32_t stack);
 /* Passes of the mapping runs since creation or the last reset that were
  * redone on the reference tree: *undecided for a hit whose reachability the
  * wide traversal could not decide, *overflow for a full stack.  Waits for

@@ -4,7 +4,7 @@ coords and hit ids bit for bit, with and without back-face culling.  A flat
 grid forces the way back to the reference tree: a leaf box of zero height is
 entered exactly at the hit's t, so whether the reference reached the leaf
 depends on its visit order, and every such pass is redone on the reference
-tree (a full stack takes the same redo)."""
+tree; a stack held to a few entries forces the other redo, for a full stack."""
 import numpy as np
 import pytest
 
@@ -20,9 +20,9 @@ WORKLOADS = {"sphere512": "sphere512",
              "knot": fa.configs.scaled("knot4096", size=512, high_quads=256)}
 
 
-def _map(tex, scene, cull, traversal):
+def _map(tex, scene, cull, traversal, stack=0):
     mm = fa.MeshMapping().init(tex, scene, cull)
-    mm.set_traversal(traversal)
+    mm.set_traversal(traversal, stack)
     mm.run_all()
     mm.reset()
     ms = mm.run_all()
@@ -73,3 +73,16 @@ def test_flat_leaf_boxes_redo_on_the_reference_tree(gpu, cull):
     assert r1["undecided"] > 0
     with pytest.raises(fa.FornosError):
         fa.MeshMapping().init(tex, scene, cull).set_traversal(2)
+
+
+@pytest.mark.parametrize("cull", [True, False])
+def test_full_stack_redoes_on_the_reference_tree(gpu, cull):
+    b = Bake(WORKLOADS["knot"])
+    c0, t0, _, _ = _map(b.prep["texels"], b.scene, cull, fa.MeshMapping.TRAVERSAL_REFERENCE)
+    c1, t1, r1, _ = _map(b.prep["texels"], b.scene, cull, fa.MeshMapping.TRAVERSAL_WIDE, stack=3)
+    print("knot cull=%d, 3-entry stack: redone passes %s" % (cull, r1))
+    assert r1["overflow"] > 0
+    assert np.array_equal(t0, t1) and same_bits(c0, c1)
+    for bad in (1, 2, 25):
+        with pytest.raises(fa.FornosError):
+            fa.MeshMapping().init(b.prep["texels"], b.scene, cull).set_traversal(fa.MeshMapping.TRAVERSAL_WIDE, bad)
