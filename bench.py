@@ -190,6 +190,7 @@ def cpu_baseline(prep, mm, solver, budget_s=15.0):
         if t > budget_s:
             break
     base = {"value": rays / t, "unit": "rays/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement, not llvmpipe (no headless GL context on the GPU box)",
             "sample": "%d texels (%d-texel chunks spread over the map) x %d AO samples = %d rays, %.1f s"
                       % (texels, 256, S, rays, t)}
     parity = {"texels": texels, "rays": rays, "count_mismatches": count_mismatch,
