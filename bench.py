@@ -6,7 +6,10 @@ heightfield, AO at 256 samples), one MI355X.
 
 A step is one AO-256 pass over every texel of the map (~1.07e9 rays, four
 launches of the hemisphere kernel), with the scene, texel set, mapping result
-and sample table already resident in HBM.  Prints one JSON line, with beside
+and sample table already resident in HBM.  Before the warm-up, the solver's
+launch shape is autotuned on 5 % of the pass (fornos_solver_autotune, reported
+with its time under config.launch_shape; --launch B,R,S fixes it instead, as
+the profiles do; --no-tune keeps the default).  Prints one JSON line, with beside
 the headline: the 8.4M-triangle knot workload's AO-512 pass (`workloads`),
 the 64 batched jobs against 64 sequential passes (`batch`), the full bake, the
 denoised bake, and the oracle on a bounded sample of the headline pass (CPU
@@ -32,9 +35,11 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = "heightfield2048"
 HBM_PEAK_GBS = 8000.0
-# published fp32 vector peak: 256 CUs x 2.4 GHz x 128 FMA lanes x 2 flops; the
-# issue roofline counts every wave64 VALU instruction as a full-width FMA
-# (128 flops), so its fraction is the VALU issue rate over the published one
+# published fp32 vector peak: 256 CUs x 2.4 GHz x 128 FMA lanes x 2 flops, a
+# figure that counts packed fp32 (two FMAs per lane per instruction), which this
+# build does not use; the issue roofline counts every wave64 VALU instruction as
+# a full-width FMA (128 flops), so its fraction is the VALU issue rate over the
+# published one, and non-packed code tops out near half of it
 VALU_PEAK_TFLOPS = 157.3
 FLOPS_PER_VALU_INST = 128
 # L2-level model of a pass's fetches: per texel, per ray, per box test and per
