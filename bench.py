@@ -110,7 +110,7 @@ def ao_pass(fa, workload, steps, warmup, log, tune=True, launch=None):
            "wall": wall, "steps": steps, "ms_per_step": 1000.0 * wall / steps,
            "value": live * S * steps / wall, "launches": launches,
            "launch_ms": float(np.mean(rays_ms)) / launches, "reduce_ms": float(np.mean(reduce_ms)),
-           "pass_kernel_ms": float(np.mean(pass_ms)), "launch_shape": tuning or {"blocks_per_cu": 8, "refill": 48,
+           "pass_kernel_ms": float(np.mean(pass_ms)), "launch_shape": tuning or {"blocks_per_cu": 8, "refill": 32,
                                                                                  "stack": 16, "tuned": False},
            "ray_order": ("Morton code of the texels' hit positions"
                          if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel set order")}

@@ -416,7 +416,7 @@ class Solver(_Handle):
         check(lib().fornos_solver_fallback_flags(self._h, ptr(out)))
         return out
 
-    def set_launch(self, blocks_per_cu=8, refill=48, stack=16):
+    def set_launch(self, blocks_per_cu=8, refill=32, stack=16):
         check(lib().fornos_solver_set_launch(self._h, int(blocks_per_cu), int(refill), int(stack)))
 
     def get_launch(self):

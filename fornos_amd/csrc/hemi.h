@@ -2,9 +2,11 @@
 //
 // k_rays runs persistent waves.  Each lane owns one ray: the ray basis of
 // Shaders/ao_step0.comp and one sample of ao/bentnormals/thick_step1.comp.
-// A lane whose ray has finished takes a new one from a global counter, one
-// atomic per wave for all of its idle lanes, so a wave never idles behind its
-// slowest ray.  Traversal is "while-while": every lane walks the 4-wide
+// A wave owns one texel at a time and hands its samples, in their
+// direction-sorted order, to its idle lanes once enough are idle; when they
+// run out it takes the next texel from a global counter (one atomic per
+// texel).  So a wave never idles behind its slowest ray, and its lanes trace
+// rays from one or two origins, which walk the tree in step.  Traversal is "while-while": every lane walks the 4-wide
 // culling tree until it holds a leaf (or is done), then the wave tests
 // triangles together, so the triangle code never runs for one lane while the
 // others wait on nodes.
@@ -48,7 +50,7 @@ enum { kWideTree = 0, kCullTree = 1, kRefTree = 2 };  // traversal modes of a ra
 constexpr uint32_t kStats = 12;
 // counting run, per texel: which fallback paths its rays took
 enum : uint32_t { kFbOverflow = 1u, kFbRedo = 2u, kFbSlow = 4u };
-constexpr uint32_t kRefill = 48;  // default: refill a wave once this many lanes are idle
+constexpr uint32_t kRefill = 32;  // default: refill a wave once this many lanes are idle
 // Leaf step once the lanes still searching are at most the lanes holding a
 // leaf / quorum (0: wait for all).  Thickness lanes that hold a leaf do not
 // walk on (its pruning needs the hits first), so they idle while others
@@ -335,6 +337,19 @@ __device__ inline void binary_step(const RayArgs& A, Lane& L, uint32_t& cn, uint
 // of divergent node and triangle fetches is hidden by more waves -- 8-16 %
 // faster than the 6-7 waves the unconstrained register allocation leaves
 // room for.
+// A job's arguments, word by word through readfirstlane: every wave reads
+// the same job, and saying so keeps them in scalar registers.
+__device__ inline RayArgs load_job(const RayArgs* jobs, uint32_t j)
+{
+    static_assert(sizeof(RayArgs) % 4 == 0, "RayArgs is whole words");
+    RayArgs A;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(jobs + j);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&A);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(RayArgs) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return A;
+}
+
 template <int STK>
 constexpr int waves_for_stack()
 {
@@ -350,8 +365,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
 {
     __shared__ uint32_t stk_node[STK * kBlock];
     uint32_t job = 0;
-    RayArgs A = BATCH ? jobs[0] : A0;
-    uint32_t total = A.ntex * A.S;
+    RayArgs A = BATCH ? load_job(jobs, 0) : A0;
+    // the wave's texel (launch index) and the next of its samples to hand out
+    // (ws == S: none left); texels are taken from the launch's counter
+    uint32_t wq = 0, ws = A.S;
     const uint32_t lane = threadIdx.x & 63;
     Lane L;
     L.ray.o = L.ray.d = L.ray.inv = L.ray.oi = mk(0, 0, 0);
@@ -366,58 +383,78 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     uint32_t cn = 0, ct = 0, nr = 0, nov = 0, nref = 0, nslow = 0;
     unsigned long long w_steps = 0, w_lanes = 0, l_steps = 0, l_lanes = 0, r_steps = 0, r_lanes = 0;
     for (;;) {
-        // refill idle lanes, one atomic per wave
+        // refill idle lanes from the wave's texel: a wave hands its lanes
+        // the samples of one texel (rays from one origin, in the samples'
+        // direction-sorted order) and takes the next texel from the launch's
+        // counter, one atomic per texel, when they run out
         const uint64_t idle = __ballot(!L.active);
         const uint32_t nidle = __popcll(idle);
         if (!exhausted && (nidle >= A.refill || nidle == 64)) {
-            uint32_t base = 0;
             if (COUNT && lane == 0) {
                 ++r_steps;
                 r_lanes += nidle;
             }
-            if (lane == 0) base = atomicAdd(A.counter, nidle);
-            base = __builtin_amdgcn_readfirstlane(base);  // wave-uniform, so the job state stays scalar
-            if (base + nidle >= total) exhausted = true;
-            if (!L.active) {
-                const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
-                if (r < total) {
-                    const uint32_t q = r / A.S, s = r - q * A.S, g = texel_of(A.list, A.t0, q);
-                    const uint32_t t = A.tidx[g];
-                    if (t != kMiss) {
-                        const Basis B = ray_basis(A.sc, A.coords[g], t);
-                        const V3 d = KIND == kThick ? -B.d : B.d;
-                        L.ray.o = B.o;
-                        const float4 rs = A.samples[(size_t)(g % A.P) * A.S + s];
-                        L.ray.d = sample_dir(B, d, rs);
-                        L.ray.inv = mk(1.0f / L.ray.d.x, 1.0f / L.ray.d.y, 1.0f / L.ray.d.z);
-                        L.ray.oi = mk(L.ray.o.x * L.ray.inv.x, L.ray.o.y * L.ray.inv.y, L.ray.o.z * L.ray.inv.z);
-                        L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
-                                       fabsf(L.ray.d.z) >= kSlowDir);
-                        L.mode = L.ray.slow ? kRefTree : kWideTree;
-                        if (COUNT && L.ray.slow) {
-                            ++nslow;
-                            atomicOr(&A.fb[g], kFbSlow);
-                        }
-                        L.q = q;
-                        L.s = __float_as_uint(rs.w);  // the sample's index in the table
-                        L.cur = 0;
-                        L.sp = 0;
-                        L.slot = kNoChild;
-                        L.node = 0;
-                        L.best = FLT_MAX;
-                        L.bn = false;
-                        L.active = true;
-                        if (COUNT) ++nr;
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+            uint32_t my_q = kMiss, my_s = 0;
+            uint32_t filled = 0;
+            while (filled < nidle) {
+                if (ws >= A.S) {
+                    uint32_t nq = 0;
+                    if (lane == 0) nq = atomicAdd(A.counter, 1u);
+                    nq = __builtin_amdgcn_readfirstlane(nq);  // wave-uniform, so the texel state stays scalar
+                    if (nq >= A.ntex) {
+                        exhausted = true;
+                        break;
                     }
+                    wq = nq;
+                    // a texel whose mapping missed casts no rays (the load is
+                    // the same for every lane: readfirstlane keeps the state scalar)
+                    ws = __builtin_amdgcn_readfirstlane(A.tidx[texel_of(A.list, A.t0, wq)]) == kMiss ? A.S : 0;
+                    continue;
                 }
+                const uint32_t take = min(nidle - filled, A.S - ws);
+                if (!L.active && rank >= filled && rank < filled + take) {
+                    my_q = wq;
+                    my_s = ws + (rank - filled);
+                }
+                filled += take;
+                ws += take;
+            }
+            if (!L.active && my_q != kMiss) {
+                const uint32_t q = my_q, s = my_s, g = texel_of(A.list, A.t0, q);
+                const uint32_t t = A.tidx[g];
+                const Basis B = ray_basis(A.sc, A.coords[g], t);
+                const V3 d = KIND == kThick ? -B.d : B.d;
+                L.ray.o = B.o;
+                const float4 rs = A.samples[(size_t)(g % A.P) * A.S + s];
+                L.ray.d = sample_dir(B, d, rs);
+                L.ray.inv = mk(1.0f / L.ray.d.x, 1.0f / L.ray.d.y, 1.0f / L.ray.d.z);
+                L.ray.oi = mk(L.ray.o.x * L.ray.inv.x, L.ray.o.y * L.ray.inv.y, L.ray.o.z * L.ray.inv.z);
+                L.ray.slow = !(fabsf(L.ray.d.x) >= kSlowDir && fabsf(L.ray.d.y) >= kSlowDir &&
+                               fabsf(L.ray.d.z) >= kSlowDir);
+                L.mode = L.ray.slow ? kRefTree : kWideTree;
+                if (COUNT && L.ray.slow) {
+                    ++nslow;
+                    atomicOr(&A.fb[g], kFbSlow);
+                }
+                L.q = q;
+                L.s = __float_as_uint(rs.w);  // the sample's index in the table
+                L.cur = 0;
+                L.sp = 0;
+                L.slot = kNoChild;
+                L.node = 0;
+                L.best = FLT_MAX;
+                L.bn = false;
+                L.active = true;
+                if (COUNT) ++nr;
             }
         }
         if (exhausted && __ballot(L.active) == 0) {
             if (!BATCH) break;
             job = __builtin_amdgcn_readfirstlane(job + 1);
             if (job >= njobs) break;
-            A = jobs[job];
-            total = A.ntex * A.S;
+            A = load_job(jobs, job);
+            ws = A.S;
             exhausted = false;
             continue;
         }

@@ -1494,7 +1494,7 @@ fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t*
         return FORNOS_OK;
     };
     // coordinate descent: stack depth, then blocks per CU, then refill threshold
-    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {32, 40, 48, 52, 56, 60}, {16, 24, 32}};
+    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}};
     float best = FLT_MAX;
     fornos_status err = FORNOS_OK;
     for (int axis : {2, 0, 1}) {
@@ -1762,7 +1762,7 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
         return FORNOS_OK;
     };
     // coordinate descent: stack depth, then blocks per CU, then refill threshold
-    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {32, 40, 48, 56}, {16, 24, 32}};
+    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}};
     float best = FLT_MAX;
     for (int axis : {2, 0, 1}) {
         uint32_t pick = cur[axis];

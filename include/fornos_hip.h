@@ -342,10 +342,11 @@ fornos_status fornos_solver_work_counters(const fornos_solver *s, uint64_t *coun
  * No reference counterpart (the reference has one traversal). */
 fornos_status fornos_solver_fallback_flags(const fornos_solver *s, uint32_t *flags);
 /* Launch shape of the solver's ray kernel: persistent blocks of 256 lanes per
- * CU (1..16), idle lanes that trigger a wave's refill (1..64), traversal
- * stack entries per lane (16, 24 or 32; a ray that would overflow it is
- * redone on a stackless tree, so results do not depend on it); default 8, 48,
- * 16.  No reference counterpart (the shaders' dispatch shape is fixed). */
+ * CU (1..16), idle lanes that trigger a wave's refill from its texel (1..64),
+ * traversal stack entries per lane (16, 24 or 32; a ray that would overflow
+ * it is redone on a stackless tree, so results do not depend on it); default
+ * 8, 32, 16.  No reference counterpart (the shaders' dispatch shape is
+ * fixed). */
 fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu, uint32_t refill,
                                        uint32_t stack);
 fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
@@ -354,11 +355,10 @@ fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_
  * first `fraction` of its texels in the pass's order (at least 2^22 rays;
  * the paired AO + bent-normals traversal when paired) by coordinate descent
  * -- stack depth {16, 24, 32}, then blocks per CU {4, 6, 8, 12}, then the
- * refill threshold {32, 40, 48, 52, 56, 60} -- and keeps the fastest (the
- * best refill threshold differs by scene: heightfield2048 AO 56, knot4096
- * AO 40).  Results never depend on the shape; the sampled texels are left
- * with their pass results and the solver's progress is unchanged.  No
- * reference counterpart. */
+ * refill threshold {16, 24, 32, 40, 48} -- and keeps the fastest.  Results
+ * never depend on the shape; the sampled texels are left with their pass
+ * results and the solver's progress is unchanged.  No reference
+ * counterpart. */
 fornos_status fornos_solver_autotune(fornos_solver *s, float fraction, uint32_t *blocks_per_cu, uint32_t *refill,
                                      uint32_t *stack, float *best_ms);
 /* Order in which a hemisphere solver's passes hand out their texels' rays.
@@ -402,7 +402,7 @@ fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_pe
                                       uint32_t *stack);
 /* Autotuner: times launch shapes on the first `fraction` of each job's
  * texels (at least 2^22 rays of each job) by coordinate descent (stack depth {16,24,32}, then blocks per CU
- * {4,6,8,12}, then refill {32,40,48,56}) and keeps the fastest. */
+ * {4,6,8,12}, then refill {16,24,32,40,48}) and keeps the fastest. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
                                     uint32_t *refill, uint32_t *stack, float *best_ms);
 /* Runs every job once (graph launch); *ms = wall time of the graph on the
