@@ -4,9 +4,11 @@
 // Shaders/ao_step0.comp and one sample of ao/bentnormals/thick_step1.comp.
 // A wave owns one texel at a time and hands its samples, in their
 // direction-sorted order, to its idle lanes once enough are idle; when they
-// run out it takes the next texel from a global counter (one atomic per
-// texel).  So a wave never idles behind its slowest ray, and its lanes trace
-// rays from one or two origins, which walk the tree in step.  Traversal is "while-while": every lane walks the 4-wide
+// run out it takes the next texel of its batch, and a new batch of texels
+// from a global counter when that is spent (one atomic per batch: one texel
+// at 256 samples and more, 256 / S texels below, at most 32).  So a wave never idles
+// behind its slowest ray, and its lanes trace rays from few origins, which
+// walk the tree in step.  Traversal is "while-while": every lane walks the 4-wide
 // culling tree until it holds a leaf (or is done), then the wave tests
 // triangles together, so the triangle code never runs for one lane while the
 // others wait on nodes.
@@ -63,6 +65,14 @@ constexpr uint32_t quorum()
     return KIND == kThick ? 2u : (KIND == kAO ? 8u : 4u);
 }
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
+// Texels a wave takes from the launch's counter at once: about 256 rays'
+// worth (1 at S >= 256, at most 32).  One texel per atomic made every wave of
+// a 16-sample pass take a texel per 16 rays from the one counter, and the
+// pass ran at a third of the 256-sample pass's rays/s.
+__device__ inline uint32_t texel_batch(uint32_t S)
+{
+    return S >= 256 ? 1u : min(32u, 256u / max(S, 1u));
+}
 
 struct RayArgs {
     DevScene sc;
@@ -358,7 +368,12 @@ constexpr int waves_for_stack()
     return waves > 8 ? 8 : (int)waves;
 }
 
-template <int KIND, bool COUNT, bool BATCH, int STK>
+// TB: texels are taken from the launch's counter in batches (texel_batch),
+// for passes of fewer than 256 samples, with the wave's texel state in LDS;
+// otherwise one texel per atomic with the state in scalar registers, which
+// leaves the register allocation of the walk as it is (the paired AO + BN
+// pass at 256 samples ran 13 % slower with the batched hand-out's state).
+template <int KIND, bool COUNT, bool BATCH, int STK, bool TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for_stack<STK>()))) void k_rays(RayArgs A0,
                                                                                          const RayArgs* __restrict__ jobs,
                                                                                          uint32_t njobs)
@@ -366,10 +381,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     __shared__ uint32_t stk_node[STK * kBlock];
     uint32_t job = 0;
     RayArgs A = BATCH ? load_job(jobs, 0) : A0;
-    // the wave's texel (launch index) and the next of its samples to hand out
-    // (ws == S: none left); texels are taken from the launch's counter
-    uint32_t wq = 0, ws = A.S;
+    // per wave: its texel (launch index) wq and the next of its samples to
+    // hand out ws (ws >= S: none left); TB: also its batch of texels from the
+    // launch's counter, [bq, bend) not yet taken.  TB keeps the state in LDS,
+    // as it is used only at refills (the batch kernel keeps it in scalar
+    // registers: in LDS its job arguments left them and spilled).
+    constexpr bool kLdsState = TB && !BATCH;
+    uint32_t wq = 0, ws = A.S, bq = 0, bend = 0;
+    __shared__ uint32_t wstate[kBlock / 64][4];
     const uint32_t lane = threadIdx.x & 63;
+    uint32_t* const ws_of = wstate[threadIdx.x >> 6];
+    if (kLdsState && lane == 0) {
+        ws_of[0] = 0;
+        ws_of[1] = 0xffffffffu;
+        ws_of[2] = 0;
+        ws_of[3] = 0;
+    }
     Lane L;
     L.ray.o = L.ray.d = L.ray.inv = L.ray.oi = mk(0, 0, 0);
     L.ray.slow = false;
@@ -385,8 +412,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     for (;;) {
         // refill idle lanes from the wave's texel: a wave hands its lanes
         // the samples of one texel (rays from one origin, in the samples'
-        // direction-sorted order) and takes the next texel from the launch's
-        // counter, one atomic per texel, when they run out
+        // direction-sorted order) and takes the next texel of its batch when
+        // they run out, and the next batch from the launch's counter (one
+        // atomic per batch) when the batch is spent
         const uint64_t idle = __ballot(!L.active);
         const uint32_t nidle = __popcll(idle);
         if (!exhausted && (nidle >= A.refill || nidle == 64)) {
@@ -395,10 +423,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 r_lanes += nidle;
             }
             const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+            if (kLdsState) {
+                wq = __builtin_amdgcn_readfirstlane(ws_of[0]);
+                ws = __builtin_amdgcn_readfirstlane(ws_of[1]);
+                bq = __builtin_amdgcn_readfirstlane(ws_of[2]);
+                bend = __builtin_amdgcn_readfirstlane(ws_of[3]);
+            }
             uint32_t my_q = kMiss, my_s = 0;
             uint32_t filled = 0;
             while (filled < nidle) {
-                if (ws >= A.S) {
+                if (ws >= A.S && !TB) {
                     uint32_t nq = 0;
                     if (lane == 0) nq = atomicAdd(A.counter, 1u);
                     nq = __builtin_amdgcn_readfirstlane(nq);  // wave-uniform, so the texel state stays scalar
@@ -412,6 +446,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                     ws = __builtin_amdgcn_readfirstlane(A.tidx[texel_of(A.list, A.t0, wq)]) == kMiss ? A.S : 0;
                     continue;
                 }
+                if (ws >= A.S) {
+                    if (bq >= bend) {
+                        const uint32_t tb = texel_batch(A.S);
+                        uint32_t nq = 0;
+                        if (lane == 0) nq = atomicAdd(A.counter, tb);
+                        nq = __builtin_amdgcn_readfirstlane(nq);  // wave-uniform, so the texel state stays scalar
+                        if (nq >= A.ntex) {
+                            exhausted = true;
+                            break;
+                        }
+                        bq = nq;
+                        bend = min(nq + tb, A.ntex);
+                    }
+                    wq = bq++;
+                    ws = __builtin_amdgcn_readfirstlane(A.tidx[texel_of(A.list, A.t0, wq)]) == kMiss ? A.S : 0;
+                    continue;
+                }
                 const uint32_t take = min(nidle - filled, A.S - ws);
                 if (!L.active && rank >= filled && rank < filled + take) {
                     my_q = wq;
@@ -419,6 +470,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 }
                 filled += take;
                 ws += take;
+            }
+            if (kLdsState && lane == 0) {
+                ws_of[0] = wq;
+                ws_of[1] = ws;
+                ws_of[2] = bq;
+                ws_of[3] = bend;
             }
             if (!L.active && my_q != kMiss) {
                 const uint32_t q = my_q, s = my_s, g = texel_of(A.list, A.t0, q);
@@ -455,6 +512,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
             if (job >= njobs) break;
             A = load_job(jobs, job);
             ws = A.S;
+            bq = bend = 0;
+            if (kLdsState && lane == 0) {
+                ws_of[1] = 0xffffffffu;
+                ws_of[3] = 0;
+            }
             exhausted = false;
             continue;
         }

@@ -212,17 +212,31 @@ fornos_status scratch_for(int device, uint8_t** out)
 }
 
 // Launch a hemisphere kernel over texels [t0, t1).
-// k_rays with a stack of 16, 24 or 32 entries per lane
-template <int KIND, bool COUNT, bool BATCH>
-void launch_rays(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs& A, const RayArgs* jobs, uint32_t nj)
+// k_rays with a stack of 16, 24 or 32 entries per lane; tb: texels handed out
+// in batches (passes of fewer than 256 samples; not in the counting run)
+template <int KIND, bool COUNT, bool BATCH, bool TB>
+void launch_rays_tb(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs& A, const RayArgs* jobs, uint32_t nj)
 {
     if (stack == 16)
-        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 16>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 16, TB>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
     else if (stack == 32)
-        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 32>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 32, TB>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
     else
-        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 24>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
+        hipLaunchKernelGGL((k_rays<KIND, COUNT, BATCH, 24, TB>), dim3(blocks), dim3(kBlock), 0, st, A, jobs, nj);
 }
+
+template <int KIND, bool COUNT, bool BATCH>
+void launch_rays(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs& A, const RayArgs* jobs, uint32_t nj,
+                 bool tb = false)
+{
+    if (tb && !COUNT)
+        launch_rays_tb<KIND, false, BATCH, true>(stack, blocks, st, A, jobs, nj);
+    else
+        launch_rays_tb<KIND, COUNT, BATCH, false>(stack, blocks, st, A, jobs, nj);
+}
+
+// Passes of fewer samples than this take their texels in batches.
+constexpr uint32_t kTexelBatchBelow = 256;
 
 bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
 {
@@ -316,7 +330,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         if (count)
             launch_rays<KIND, true, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
         else
-            launch_rays<KIND, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
+            launch_rays<KIND, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, S < kTexelBatchBelow);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
         if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
@@ -394,7 +408,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint
             for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
             HIP_TRY(hipEventRecord(ev[0], sc->stream));
         }
-        launch_rays<kAOBN, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
+        launch_rays<kAOBN, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, S < kTexelBatchBelow);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<kAO>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Ra);
         hipLaunchKernelGGL(k_reduce<kBN>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Rb);
@@ -1567,6 +1581,7 @@ struct fornos_batch {
     uint32_t* counters = nullptr;
     int cus = 256;
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;
+    bool texel_batches = false;  // some job has fewer than kTexelBatchBelow samples
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     ~fornos_batch()
@@ -1600,7 +1615,7 @@ fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, R
     uint64_t rays = 0;
     for (const RayArgs& J : jobs) rays += (uint64_t)J.ntex * J.S;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)b->cus * b->blocks_per_cu, div_up(rays, kBlock));
-    if (blocks) launch_rays<kAO, false, true>(b->stack, blocks, st, jobs[0], djobs, nj);
+    if (blocks) launch_rays<kAO, false, true>(b->stack, blocks, st, jobs[0], djobs, nj, b->texel_batches);
     for (uint32_t j = 0; j < nj; ++j) {
         const fornos_solver* s = b->solvers[j];
         ReduceArgs R{};
@@ -1677,6 +1692,7 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         if (fornos_status st = pass_order(s, &list)) return st;
         A.list = list;
         b->jobs.push_back(A);
+        if (A.S < kTexelBatchBelow) b->texel_batches = true;
     }
     HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * n, hipMemcpyHostToDevice));
     *out = b.release();
