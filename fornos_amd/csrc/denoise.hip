@@ -29,6 +29,8 @@ using namespace fornos::dn;
 
 namespace {
 
+static_assert(FORNOS_DN_IN <= 16 && FORNOS_DN_IN % 4 == 0, "the first layer reads at most 16 fp32 channels, 4 at a time");
+
 struct Layer {
     uint32_t c1, c2, cout, up, level;
     uint64_t w_off, b_off;  // in the parameter vector
@@ -43,10 +45,20 @@ void launch_conv(const ConvArgs& A, hipStream_t st)
     constexpr int MT = 4, TH = 2 * MT, TW = 32;
     const uint32_t tx = div_up(A.w, TW), ty = div_up(A.h, TH);
     const uint32_t tiles = A.n * tx * ty;
-    if (A.cout % 64 == 0)
-        hipLaunchKernelGGL((k_conv3x3<MT, 4>), dim3(tiles, A.cout / 64), dim3(kThreads), 0, st, A, tx, ty);
+    const dim3 grid32(tiles, A.cout / 32), grid64(tiles, A.cout / 64);
+    if (A.in_f32)  // the first layer (32 output channels)
+        hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseF32In>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+    else if (A.head_out)  // the last layer (32 output channels)
+        hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseHead>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+    else if (A.cout % 64 == 0)
+        if (A.pool)
+            hipLaunchKernelGGL((k_conv3x3<MT, 4, kFusePool>), grid64, dim3(kThreads), 0, st, A, tx, ty);
+        else
+            hipLaunchKernelGGL((k_conv3x3<MT, 4>), grid64, dim3(kThreads), 0, st, A, tx, ty);
+    else if (A.pool)
+        hipLaunchKernelGGL((k_conv3x3<MT, 2, kFusePool>), grid32, dim3(kThreads), 0, st, A, tx, ty);
     else
-        hipLaunchKernelGGL((k_conv3x3<MT, 2>), dim3(tiles, A.cout / 32), dim3(kThreads), 0, st, A, tx, ty);
+        hipLaunchKernelGGL((k_conv3x3<MT, 2>), grid32, dim3(kThreads), 0, st, A, tx, ty);
 }
 
 // k_wgrad + k_wgrad_reduce; `part` is their partial-tile buffer (grown when
@@ -99,6 +111,7 @@ struct fornos_denoiser {
     uint64_t n_params = 0, n_flip = 0, head_w = 0, head_b = 0;
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *wb = nullptr, *wflip = nullptr;
+    uint16_t* wfirst = nullptr;  // the first layer's weights packed for the fp32-input kernel
     uint64_t step = 0;
     uint64_t cap_pixels = 0;
     uint16_t* x0 = nullptr;
@@ -113,7 +126,9 @@ struct fornos_denoiser {
     ~fornos_denoiser()
     {
         (void)hipDeviceSynchronize();
-        for (void* p : {(void*)params, (void*)grads, (void*)m, (void*)v, (void*)wb, (void*)wflip, (void*)wpart}) dev_free(p);
+        for (void* p : {(void*)params, (void*)grads, (void*)m, (void*)v, (void*)wb, (void*)wflip, (void*)wfirst,
+                        (void*)wpart})
+            dev_free(p);
         free_acts();
     }
     void free_acts()
@@ -213,7 +228,7 @@ struct fornos_denoiser {
         }
     }
 
-    void conv_forward(uint32_t i, uint32_t n, uint32_t h, uint32_t w)
+    void conv_forward(uint32_t i, uint32_t n, uint32_t h, uint32_t w, const float* in, float* o, bool train)
     {
         const Layer& L = layers[i];
         ConvArgs A{};
@@ -230,26 +245,45 @@ struct fornos_denoiser {
         A.w = w >> L.level;
         A.up_a = (int)L.up;
         A.relu = 1;
+        if (i == 0 && !train) {
+            // the network input straight from the fp32 image, with the
+            // weights packed for it (training keeps the packed copy of the
+            // image: the weight gradient of this layer reads it)
+            A.in_f32 = in;
+            A.in_c = FORNOS_DN_IN;
+            A.wts = wfirst;
+        }
+        if (i < 2 * levels && (i & 1) && i / 2 + 1 < levels) A.pool = pool[i / 2 + 1];  // input of the next level
+        if (i == dec(0, 1)) {
+            A.head_in = in;
+            A.head_cin = FORNOS_DN_IN;
+            A.head_w = params + head_w;
+            A.head_b = params + head_b;
+            A.head_out = o;
+            if (!train) A.out = nullptr;  // the head's backward reads these in training
+        }
         launch_conv(A, st);
     }
 
-    void forward(const float* in, uint32_t n, uint32_t h, uint32_t w, float* o)
+    // train: keep what the backward pass reads (the packed input and the
+    // last layer's activations); inference reads the fp32 image in the
+    // first layer and stores only the head's output from the last.
+    void forward(const float* in, uint32_t n, uint32_t h, uint32_t w, float* o, bool train)
     {
         const uint64_t P = (uint64_t)n * h * w;
-        launch_1d(k_pack_input, P * 4, st, in, x0, (size_t)P, (uint32_t)FORNOS_DN_IN);
+        if (train)
+            launch_1d(k_pack_input, P * 4, st, in, x0, (size_t)P, (uint32_t)FORNOS_DN_IN);
+        else  // from the current bf16 weights (training updates them)
+            launch_1d(k_pack_first, (size_t)layers[0].cout * 160, st, (const uint16_t*)(wb + layers[0].w_off), wfirst,
+                      layers[0].cout, layers[0].c1);
         for (uint32_t l = 0; l < levels; ++l) {
-            if (l > 0)
-                launch_1d(k_pool, (P >> (2 * l)) * ch(l - 1) / 8, st, (const uint16_t*)act[enc(l - 1, 1)], pool[l],
-                          n, h >> (l - 1), w >> (l - 1), ch(l - 1));
-            conv_forward(enc(l, 0), n, h, w);
-            conv_forward(enc(l, 1), n, h, w);
+            conv_forward(enc(l, 0), n, h, w, in, o, train);
+            conv_forward(enc(l, 1), n, h, w, in, o, train);
         }
         for (int l = (int)levels - 2; l >= 0; --l) {
-            conv_forward(dec(l, 0), n, h, w);
-            conv_forward(dec(l, 1), n, h, w);
+            conv_forward(dec(l, 0), n, h, w, in, o, train);
+            conv_forward(dec(l, 1), n, h, w, in, o, train);
         }
-        launch_1d(k_head, P, st, (const uint16_t*)act[dec(0, 1)], in, (uint32_t)FORNOS_DN_IN,
-                  (const float*)(params + head_w), (const float*)(params + head_b), o, (size_t)P);
     }
 
     // weight gradient of layer i, then the gradient of its inputs
@@ -374,6 +408,7 @@ fornos_status fornos_denoiser_create(uint32_t levels, uint32_t base_channels, ui
         if (fornos_status s = dev_alloc(p, d->n_params)) return s;
     if (fornos_status s = dev_alloc(&d->wb, d->n_params)) return s;
     if (fornos_status s = dev_alloc(&d->wflip, d->n_flip)) return s;
+    if (fornos_status s = dev_alloc(&d->wfirst, (size_t)d->layers[0].cout * 160)) return s;
     HIP_TRY(hipMemcpy(d->params, h.data(), sizeof(float) * d->n_params, hipMemcpyHostToDevice));
     launch_1d(k_zero, d->n_params, d->st, d->grads, (size_t)d->n_params);
     launch_1d(k_zero, d->n_params, d->st, d->m, (size_t)d->n_params);
@@ -444,7 +479,7 @@ fornos_status fornos_denoiser_forward(fornos_denoiser* d, const float* input, ui
     if (fornos_status s = d->ensure(n, h, w)) return s;
     Timer tm(d->st);
     if (fornos_status s = tm.start()) return s;
-    d->forward(input, n, h, w, output);
+    d->forward(input, n, h, w, output, false);
     HIP_TRY(hipGetLastError());
     if (fornos_status s = tm.stop(ms)) return s;
     HIP_TRY(hipDeviceSynchronize());
@@ -459,7 +494,7 @@ fornos_status fornos_denoiser_backward(fornos_denoiser* d, const float* input, c
     if (fornos_status s = d->ensure(n, h, w)) return s;
     const uint64_t P = (uint64_t)n * h * w;
     hipStream_t st = d->st;
-    d->forward(input, n, h, w, d->out);
+    d->forward(input, n, h, w, d->out, true);
     launch_1d(k_zero_u32, 1, st, d->valid, (size_t)1);
     hipLaunchKernelGGL(k_count_valid, dim3(std::min<uint32_t>(1024, div_up(P, kThreads))), dim3(kThreads), 0, st,
                        input, (uint32_t)FORNOS_DN_IN, (uint32_t)FORNOS_DN_MASK_CHANNEL, (size_t)P, d->valid);

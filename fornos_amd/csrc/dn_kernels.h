@@ -51,16 +51,34 @@ __device__ inline bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(
 // is not positive (the ReLU derivative in the backward pass); acc2 adds into
 // out2 instead of overwriting it.  The same kernel computes the backward
 // pass with respect to the input, with weights flipped and transposed.
+//
+// Forward-pass fusions (bf16 outputs, split == cout, no mask):
+// - in_f32: the input is the network's fp32 image with in_c <= 16 channels
+//   (a multiple of 4), converted to bf16 while staged as k_pack_input would,
+//   and the weights come packed by k_pack_first ([cout][9 x 16 + 16], `a` is
+//   not read): the first layer, whose other input channels are zero;
+// - pool: the 2x2 average of the bf16 outputs is written there as well, as
+//   k_pool computes it from them (h, w even);
+// - head_out: the 1x1 head of k_head on the bf16 outputs (cout = 32, one
+//   block holds a pixel's channels), residual on head_in's channels 0..3;
+//   out may then be null (the activations are not stored).
 struct ConvArgs {
     const uint16_t* a;
     const uint16_t* b;
     const uint16_t* wts;    // [cout][9][c1 + c2]
     const float* bias;      // [cout] or null
-    void* out;              // bf16 or fp32 [pixels][split]
+    void* out;              // bf16 or fp32 [pixels][split], or null (head_out only)
     void* out2;             // bf16 [pixels][cout - split]
     const uint16_t* mask1;  // [pixels][split] or null
     const uint16_t* mask2;  // [pixels][cout - split] or null
+    const float* in_f32;    // [pixels][in_c] or null
+    uint16_t* pool;         // [pixels / 4][cout] or null
+    const float* head_in;   // [pixels][head_cin]
+    const float* head_w;    // [4][32]
+    const float* head_b;    // [4]
+    float* head_out;        // [pixels][4] or null
     uint32_t c1, c2, cout, split, n, h, w;
+    uint32_t in_c, head_cin;
     int up_a, relu, out_f32, acc2;
 };
 
@@ -79,13 +97,19 @@ struct ConvArgs {
 // tap's output channel) sits at word 4p + q.
 __device__ inline int lds_word(int p, int q) { return 4 * p + q; }
 
-template <int MT, int NT>
+// FUSE: which of the fusions the kernel is built with (kFuse*; the
+// arguments select them too), so the kernels without them keep their
+// register allocation.
+enum { kFuseF32In = 1, kFusePool = 2, kFuseHead = 4 };
+template <int MT, int NT, int FUSE = 0>
 __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles_x, uint32_t tiles_y)
 {
     constexpr int TW = 32, TH = 2 * MT, IW = TW + 2, IH = TH + 2, NB = 16 * NT;
     // [pixel][4 x 8 channels] and [tap][cout][4 x 8 channels]; after the last
     // slice, the block's bf16 outputs [pixel][NB channels] (forward pass)
-    constexpr int NLDS = IH * IW * 4 + 9 * NB * 4;
+    // (the first layer stages 2 words per pixel and [NB][20] words of weights)
+    constexpr int NLDS_IN = (FUSE & kFuseF32In) ? IH * IW * 2 + NB * 20 : IH * IW * 4 + 9 * NB * 4;
+    constexpr int NLDS = NLDS_IN > TH * TW * NB / 8 ? NLDS_IN : TH * TW * NB / 8;
     static_assert(TH * TW * NB * 2 <= NLDS * 16, "output tile fits the staging LDS");
     __shared__ uint4 s_lds[NLDS];
     uint4* const s_in = s_lds;
@@ -140,6 +164,60 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
                              : make_uint4(0, 0, 0, 0);
         }
     };
+    if constexpr ((FUSE & kFuseF32In) != 0) {
+        // first layer: the fp32 image's in_c <= 16 channels as 16 bf16 (two
+        // 16-byte words per pixel) and K = 9 taps x 16 channels, padded to 160:
+        // K-step s, lane group kq holds K-group g = 4s + kq = (tap g / 2,
+        // channels 8 (g & 1) .. + 7), so 5 K-steps cover what 9 slice-taps of
+        // 32 zero-padded channels did.  Weights [cout][160] (k_pack_first).
+        static_assert(NT == 2, "the first layer has 32 output channels");
+        constexpr int KW = 20;  // 16-byte words of K per output channel
+        static_assert(IH * IW * 2 + NB * KW <= NLDS, "the first layer's staging fits");
+        uint4* const f_in = s_lds;
+        uint4* const f_wt = s_lds + IH * IW * 2;
+        for (int i = tid; i < IH * IW * 2; i += kThreads) {
+            const int h = i & 1, pix = i >> 1;
+            const int iy = pix / IW, ix = pix - iy * IW;
+            const int y = y0 + iy - 1, x = x0 + ix - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < H && x >= 0 && x < W) {
+                const float* px = A.in_f32 + (((size_t)img * H + y) * W + x) * A.in_c;
+                uint16_t* v8 = reinterpret_cast<uint16_t*>(&v);
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    const uint32_t c = (uint32_t)(8 * h + 4 * g);
+                    if (c < A.in_c) {
+                        const float4 f = *reinterpret_cast<const float4*>(px + c);
+                        v8[4 * g] = f2bf(f.x);
+                        v8[4 * g + 1] = f2bf(f.y);
+                        v8[4 * g + 2] = f2bf(f.z);
+                        v8[4 * g + 3] = f2bf(f.w);
+                    }
+                }
+            }
+            f_in[i] = v;
+        }
+        for (int i = tid; i < NB * KW; i += kThreads)
+            f_wt[i] = *reinterpret_cast<const uint4*>(A.wts + (size_t)(cbase + i / KW) * (8 * KW) + 8 * (i % KW));
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 5; ++ks) {
+            const int g = 4 * ks + kq, t = g >> 1, h = g & 1;
+            const int dy = t / 3, dx = t - 3 * (t / 3);
+            bf16x8 bw[NT];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) bw[j] = __builtin_bit_cast(bf16x8, f_wt[(j * 16 + r) * KW + 4 * ks + kq]);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const int row = wave * (MT / 2) + (m >> 1), col = (m & 1) * 16 + r;
+                const bf16x8 af = t < 9 ? __builtin_bit_cast(bf16x8, f_in[2 * ((row + dy) * IW + col + dx) + h]) : zero8();
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw[j], acc[m][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // the LDS reads are done before the epilogue stages the outputs
+    } else {
     fetch(0);
     for (uint32_t c0 = 0; c0 < cin; c0 += 32) {
 #pragma unroll
@@ -171,12 +249,27 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
         }
         __syncthreads();  // the slice's LDS reads are done before the next is staged
     }
+    }
     // D[m][n]: lane holds pixels 4*kq .. 4*kq + 3 of its fragment, output channel r
     if (!A.out_f32 && A.split == A.cout && !A.mask1) {
         // forward pass: bf16 outputs through LDS, then 16-byte stores of 8
         // consecutive channels of a pixel (the LDS reads of the last slice are
         // done: the loop ends with a barrier)
         uint16_t* s_out = reinterpret_cast<uint16_t*>(s_lds);
+        // head: this thread's pixel's residual channels and the head's
+        // weights (to LDS past the outputs), loaded before the staging so
+        // their latency overlaps it
+        float* s_head = reinterpret_cast<float*>(s_lds) + TH * TW * NB / 2;
+        float4 hin = make_float4(0, 0, 0, 0);
+        const bool head = (FUSE & kFuseHead) && NB == 32 && A.head_out;
+        static_assert(TH * TW == kThreads, "the head takes one pixel per thread");
+        static_assert(TH * TW * NB * 2 + 132 * 4 <= NLDS * 16, "the head's weights fit past the outputs");
+        if (head) {
+            const int y = y0 + tid / TW, x = x0 + tid % TW;
+            if (y < H && x < W) hin = *reinterpret_cast<const float4*>(A.head_in + (((size_t)img * H + y) * W + x) * A.head_cin);
+            if (tid < 128) s_head[tid] = A.head_w[tid];
+            else if (tid < 132) s_head[tid] = A.head_b[tid - 128];
+        }
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
             const int cl = j * 16 + r;
@@ -194,13 +287,60 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
         }
         __syncthreads();
         constexpr int CH = NB / 8;  // 16-byte chunks per pixel
-        uint16_t* out = static_cast<uint16_t*>(A.out);
-        for (int i = tid; i < TH * TW * CH; i += kThreads) {
-            const int pix = i / CH, c = i - pix * CH;
-            const int y = y0 + pix / TW, x = x0 + pix % TW;
-            if (y >= H || x >= W) continue;
-            *reinterpret_cast<uint4*>(out + (((size_t)img * H + y) * W + x) * A.cout + cbase + 8 * c) =
-                *reinterpret_cast<const uint4*>(s_out + pix * NB + 8 * c);
+        if (A.out) {
+            uint16_t* out = static_cast<uint16_t*>(A.out);
+            for (int i = tid; i < TH * TW * CH; i += kThreads) {
+                const int pix = i / CH, c = i - pix * CH;
+                const int y = y0 + pix / TW, x = x0 + pix % TW;
+                if (y >= H || x >= W) continue;
+                *reinterpret_cast<uint4*>(out + (((size_t)img * H + y) * W + x) * A.cout + cbase + 8 * c) =
+                    *reinterpret_cast<const uint4*>(s_out + pix * NB + 8 * c);
+            }
+        }
+        if ((FUSE & kFusePool) && A.pool) {
+            // k_pool's sum: the four bf16 outputs of a 2x2 cell in row-major
+            // order, times 0.25 (the tile's origin is even)
+            const int Hh = H >> 1, Wh = W >> 1;
+            for (int i = tid; i < (TH / 2) * (TW / 2) * CH; i += kThreads) {
+                const int pp = i / CH, c = i - pp * CH;
+                const int py = pp / (TW / 2), px = pp - py * (TW / 2);
+                const int gy = (y0 >> 1) + py, gx = (x0 >> 1) + px;
+                if (gy >= Hh || gx >= Wh) continue;
+                float sm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int pix = (2 * py + (k >> 1)) * TW + 2 * px + (k & 1);
+                    const uint4 v = *reinterpret_cast<const uint4*>(s_out + pix * NB + 8 * c);
+                    const uint16_t* v8 = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) sm[j] += bf2f(v8[j]);
+                }
+                uint4 o;
+                uint16_t* o8 = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) o8[j] = f2bf(sm[j] * 0.25f);
+                *reinterpret_cast<uint4*>(A.pool + (((size_t)img * Hh + gy) * Wh + gx) * A.cout + cbase + 8 * c) = o;
+            }
+        }
+        if (head) {
+            // k_head's sums for this thread's pixel, in its order
+            const int pix = tid, y = y0 + pix / TW, x = x0 + pix % TW;
+            if (y < H && x < W) {
+                const size_t p = ((size_t)img * H + y) * W + x;
+                float sm[4] = {hin.x + s_head[128], hin.y + s_head[129], hin.z + s_head[130], hin.w + s_head[131]};
+#pragma unroll
+                for (int c = 0; c < 32; c += 8) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(s_out + pix * NB + c);
+                    const uint16_t* v8 = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float yv = bf2f(v8[j]);
+#pragma unroll
+                        for (int o = 0; o < 4; ++o) sm[o] += s_head[o * 32 + c + j] * yv;
+                    }
+                }
+                *reinterpret_cast<float4*>(A.head_out + p * 4) = make_float4(sm[0], sm[1], sm[2], sm[3]);
+            }
         }
         return;
     }
@@ -603,6 +743,18 @@ __global__ void k_flip_weights(const uint16_t* w, uint16_t* wt, uint32_t cout, u
     const uint32_t t = (uint32_t)((i / cin) % 9);
     const uint32_t co = (uint32_t)(i / ((size_t)cin * 9));
     wt[((size_t)ci * 9 + (8 - t)) * cout + co] = w[i];
+}
+
+// The first layer's weights [cout][9][cin] (bf16) as [cout][160]: K index
+// 16 t + c holds tap t, channel c < 16; the rest is zero (k_conv3x3 with
+// kFuseF32In).
+__global__ void k_pack_first(const uint16_t* w, uint16_t* wk, uint32_t cout, uint32_t cin)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)cout * 160) return;
+    const uint32_t k = (uint32_t)(i % 160), co = (uint32_t)(i / 160);
+    const uint32_t t = k >> 4, c = k & 15;
+    wk[i] = (t < 9 && c < cin) ? w[((size_t)co * 9 + t) * cin + c] : (uint16_t)0;
 }
 
 __global__ void k_zero(float* p, size_t n)

@@ -405,7 +405,8 @@ def time_network(net, n, h, w, reps=10):
     x = torch.rand((n, h, w, IN), dtype=torch.float32, device="cuda")
     x[..., MASK] = 1.0
     y = torch.rand((n, h, w, OUT), dtype=torch.float32, device="cuda")
-    net.forward(x)
+    for _ in range(3):  # warm-up (allocations, clocks)
+        net.forward(x)
     fw = [net.forward(x)[1] for _ in range(reps)]
     net.backward(x, y)
     torch.cuda.synchronize()
