@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
         float* s_head = reinterpret_cast<float*>(s_lds) + TH * TW * NB / 2;
         float4 hin = make_float4(0, 0, 0, 0);
         const bool head = (FUSE & kFuseHead) && NB == 32 && A.head_out;
-        static_assert(TH * TW == kThreads, "the head takes one pixel per thread");
+        static_assert(!(FUSE & kFuseHead) || TH * TW == kThreads, "the head takes one pixel per thread");
         static_assert(TH * TW * NB * 2 + 132 * 4 <= NLDS * 16, "the head's weights fit past the outputs");
         if (head) {
             const int y = y0 + tid / TW, x = x0 + tid % TW;
