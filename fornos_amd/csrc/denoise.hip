@@ -46,17 +46,28 @@ void launch_conv(const ConvArgs& A, hipStream_t st)
     const uint32_t tx = div_up(A.w, TW), ty = div_up(A.h, TH);
     const uint32_t tiles = A.n * tx * ty;
     const dim3 grid32(tiles, A.cout / 32), grid64(tiles, A.cout / 64);
+    const bool one = A.c1 + A.c2 == 32;  // one input slice: no next-slice prefetch registers
     if (A.in_f32)  // the first layer (32 output channels)
         hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseF32In>), grid32, dim3(kThreads), 0, st, A, tx, ty);
     else if (A.head_out)  // the last layer (32 output channels)
-        hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseHead>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+        if (one)
+            hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseHead | kOneSlice>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+        else
+            hipLaunchKernelGGL((k_conv3x3<MT, 2, kFuseHead>), grid32, dim3(kThreads), 0, st, A, tx, ty);
     else if (A.cout % 64 == 0)
         if (A.pool)
             hipLaunchKernelGGL((k_conv3x3<MT, 4, kFusePool>), grid64, dim3(kThreads), 0, st, A, tx, ty);
+        else if (one)
+            hipLaunchKernelGGL((k_conv3x3<MT, 4, kOneSlice>), grid64, dim3(kThreads), 0, st, A, tx, ty);
         else
             hipLaunchKernelGGL((k_conv3x3<MT, 4>), grid64, dim3(kThreads), 0, st, A, tx, ty);
     else if (A.pool)
-        hipLaunchKernelGGL((k_conv3x3<MT, 2, kFusePool>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+        if (one)
+            hipLaunchKernelGGL((k_conv3x3<MT, 2, kFusePool | kOneSlice>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+        else
+            hipLaunchKernelGGL((k_conv3x3<MT, 2, kFusePool>), grid32, dim3(kThreads), 0, st, A, tx, ty);
+    else if (one)
+        hipLaunchKernelGGL((k_conv3x3<MT, 2, kOneSlice>), grid32, dim3(kThreads), 0, st, A, tx, ty);
     else
         hipLaunchKernelGGL((k_conv3x3<MT, 2>), grid32, dim3(kThreads), 0, st, A, tx, ty);
 }

@@ -100,9 +100,20 @@ __device__ inline int lds_word(int p, int q) { return 4 * p + q; }
 // FUSE: which of the fusions the kernel is built with (kFuse*; the
 // arguments select them too), so the kernels without them keep their
 // register allocation.
-enum { kFuseF32In = 1, kFusePool = 2, kFuseHead = 4 };
+enum { kFuseF32In = 1, kFusePool = 2, kFuseHead = 4, kOneSlice = 8 };  // kOneSlice: cin = 32
+// Waves per SIMD the kernel is held to: 5 for the first layer (21 KB of
+// LDS per block; 6 spills), 4 for the other single-slice 32-channel layers
+// (no next-slice prefetch registers, and their LDS allows 4 blocks per CU),
+// else the compiler's choice (64 output channels: LDS allows 2 blocks per CU
+// either way).  Forward 2.006 -> 1.91 ms.
+template <int NT, int FUSE>
+constexpr int conv_waves()
+{
+    return (FUSE & kFuseF32In) ? 5 : (NT == 2 && (FUSE & kOneSlice)) ? 4 : 1;
+}
 template <int MT, int NT, int FUSE = 0>
-__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles_x, uint32_t tiles_y)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(conv_waves<NT, FUSE>()))) void k_conv3x3(
+    ConvArgs A, uint32_t tiles_x, uint32_t tiles_y)
 {
     constexpr int TW = 32, TH = 2 * MT, IW = TW + 2, IH = TH + 2, NB = 16 * NT;
     // [pixel][4 x 8 channels] and [tap][cout][4 x 8 channels]; after the last
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs A, uint32_t tiles
             if (i < NWT) s_wt[lds_word(i >> 2, i & 3)] = rwt[k];
         }
         __syncthreads();
-        if (c0 + 32 < cin) fetch(c0 + 32);
+        if (!(FUSE & kOneSlice) && c0 + 32 < cin) fetch(c0 + 32);
 #pragma unroll 3
         for (int t = 0; t < 9; ++t) {
             const int dy = t / 3, dx = t - 3 * (t / 3);  // offsets in the haloed tile
