@@ -7,9 +7,10 @@ heightfield, AO at 256 samples), one MI355X.
 A step is one AO-256 pass over every texel of the map (~1.07e9 rays, four
 launches of the hemisphere kernel), with the scene, texel set, mapping result
 and sample table already resident in HBM.  Before the warm-up, the solver's
-launch shape is autotuned on 5 % of the pass (fornos_solver_autotune, reported
-with its time under config.launch_shape; --launch B,R,S fixes it instead, as
-the profiles do; --no-tune keeps the default).  Prints one JSON line, with beside
+launch shape, hand-out tile and ray order are autotuned on 5 % of the pass
+(fornos_solver_autotune, reported with its time under config.launch_shape and
+config.ray_order; --launch B,R,S fixes the shape with the default tile and
+order instead, as the profiles do; --no-tune keeps the defaults).  Prints one JSON line, with beside
 the headline: the 8.4M-triangle knot workload's AO-512 pass (`workloads`),
 the 64 batched jobs against 64 sequential passes (`batch`), the full bake, the
 denoised bake, and the oracle on a bounded sample of the headline pass (CPU
@@ -81,18 +82,23 @@ def ao_pass(fa, workload, steps, warmup, log, tune=True, launch=None):
     live = int(np.sum(tidx != 0xFFFFFFFF))
     S = p["sampleCount"]
     tuning = None
+    auto_order = ao.effective_ray_order()  # the order the profiles were taken in (the automatic one)
     if launch:
         ao.set_launch(*launch)
-        tuning = {"blocks_per_cu": launch[0], "refill": launch[1], "stack": launch[2], "tuned": False}
+        tuning = {"blocks_per_cu": launch[0], "refill": launch[1], "stack": launch[2], "tile": ao.get_tile(),
+                  "tuned": False}
     elif tune:
-        # launch shape timed on 5 % of the pass (results never depend on it);
-        # its cost is set-up, outside the timed steps, and reported
+        # launch shape, hand-out tile and ray order timed on 5 % of the pass
+        # (results never depend on them); the cost is set-up, outside the
+        # timed steps, and reported
         t0 = time.perf_counter()
         bpc, refill, stack, tms = ao.autotune(0.05)
-        tuning = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "tuned": True,
+        tuning = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "tile": ao.get_tile(), "tuned": True,
                   "seconds": time.perf_counter() - t0, "fraction": 0.05}
-        log("# %s autotuned launch shape: %d blocks/CU, refill at %d idle lanes, stack %d (%.2f s)"
-            % (workload, bpc, refill, stack, tuning["seconds"]))
+        log("# %s autotuned launch: %d blocks/CU, refill at %d idle lanes, stack %d, %d texel(s) per hand-out, "
+            "%s order (%.2f s)" % (workload, bpc, refill, stack, tuning["tile"],
+                                   "Morton" if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel",
+                                   tuning["seconds"]))
     for _ in range(warmup):
         ao.reset()
         ao.run_all()
@@ -111,9 +117,12 @@ def ao_pass(fa, workload, steps, warmup, log, tune=True, launch=None):
            "value": live * S * steps / wall, "launches": launches,
            "launch_ms": float(np.mean(rays_ms)) / launches, "reduce_ms": float(np.mean(reduce_ms)),
            "pass_kernel_ms": float(np.mean(pass_ms)), "launch_shape": tuning or {"blocks_per_cu": 8, "refill": 32,
-                                                                                 "stack": 16, "tuned": False},
+                                                                                 "stack": 16, "tile": ao.get_tile(),
+                                                                                 "tuned": False},
            "ray_order": ("Morton code of the texels' hit positions"
-                         if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel set order")}
+                         if ao.effective_ray_order() == ao.RAY_ORDER_MORTON else "texel set order"),
+           "default_tile_and_order": ao.get_tile() == (1 if S >= 256 else min(32, 256 // S)) and
+                                     ao.effective_ray_order() == auto_order}
     log("# %s AO-%d: %.2f ms/step, %.3e rays/s" % (workload, S, res["ms_per_step"], res["value"]))
     return res
 
@@ -135,9 +144,12 @@ def roofline(res):
     shape = res["launch_shape"]
     pshape = prof.get("launch_shape")
     out["profile_launch_shape"] = pshape
-    if pshape and [shape["blocks_per_cu"], shape["refill"], shape["stack"]] != list(pshape):
-        out["profile_note"] = ("counters taken at launch shape %s, this run used %s"
-                               % (pshape, [shape["blocks_per_cu"], shape["refill"], shape["stack"]]))
+    if pshape and ([shape["blocks_per_cu"], shape["refill"], shape["stack"]] != list(pshape) or
+                   not res["default_tile_and_order"]):
+        out["profile_note"] = ("counters taken at launch shape %s with the default hand-out tile and the automatic "
+                               "ray order, this run used %s, tile %d, %s"
+                               % (pshape, [shape["blocks_per_cu"], shape["refill"], shape["stack"]], shape["tile"],
+                                  res["ray_order"]))
     cands = []
     if valu:
         tf = valu * FLOPS_PER_VALU_INST / t / 1e12

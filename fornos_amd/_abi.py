@@ -151,6 +151,8 @@ _SIGS = [
     ("fornos_solver_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("fornos_solver_get_launch", None, [H, u32p, u32p, u32p]),
     ("fornos_solver_set_ray_order", None, [H, C.c_int]),
+    ("fornos_solver_set_tile", None, [H, C.c_uint32]),
+    ("fornos_solver_get_tile", None, [H, u32p]),
     ("fornos_solver_autotune", None, [H, C.c_float, u32p, u32p, u32p, C.POINTER(C.c_float)]),
     ("fornos_solver_get_ray_order", None, [H, C.POINTER(C.c_int)]),
     ("fornos_solver_effective_ray_order", None, [H, C.POINTER(C.c_int)]),

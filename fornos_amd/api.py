@@ -424,10 +424,24 @@ class Solver(_Handle):
         check(lib().fornos_solver_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def set_tile(self, texels=0):
+        """Texels a wave takes from the pass's counter at once (1..32; 0:
+        256 / sampleCount, the default) (fornos_solver_set_tile).  Results do
+        not depend on it."""
+        check(lib().fornos_solver_set_tile(self._h, int(texels)))
+
+    def get_tile(self):
+        """The hand-out tile in effect (the default resolved)."""
+        t = C.c_uint32()
+        check(lib().fornos_solver_get_tile(self._h, C.byref(t)))
+        return t.value
+
     def autotune(self, fraction=0.05):
-        """Times launch shapes on the first `fraction` of the pass's texels
-        and keeps the fastest; returns (blocks_per_cu, refill, stack, ms)
-        (fornos_solver_autotune).  Results never depend on the shape."""
+        """Times launch shapes and hand-out tiles on the first `fraction` of
+        the pass's texels, then the texel and Morton ray orders on the same
+        texels, and keeps the fastest; returns (blocks_per_cu, refill, stack,
+        ms) (fornos_solver_autotune; get_tile and get_ray_order give the
+        rest).  Results never depend on any of these."""
         a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().fornos_solver_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(c), C.byref(ms)))
         return a.value, b.value, c.value, ms.value

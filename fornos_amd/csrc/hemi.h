@@ -5,8 +5,9 @@
 // A wave owns one texel at a time and hands its samples, in their
 // direction-sorted order, to its idle lanes once enough are idle; when they
 // run out it takes the next texel of its batch, and a new batch of texels
-// from a global counter when that is spent (one atomic per batch: one texel
-// at 256 samples and more, 256 / S texels below, at most 32).  So a wave never idles
+// from a global counter when that is spent (one atomic per batch of tb
+// texels, the pass's hand-out tile: by default one texel at 256 samples and
+// more, 256 / S texels below, at most 32).  So a wave never idles
 // behind its slowest ray, and its lanes trace rays from few origins, which
 // walk the tree in step.  Traversal is "while-while": every lane walks the 4-wide
 // culling tree until it holds a leaf (or is done), then the wave tests
@@ -65,14 +66,6 @@ constexpr uint32_t quorum()
     return KIND == kThick ? 2u : (KIND == kAO ? 8u : 4u);
 }
 constexpr uint32_t kStack = 16;   // default 4-wide traversal stack entries per lane (16, 24 or 32)
-// Texels a wave takes from the launch's counter at once: about 256 rays'
-// worth (1 at S >= 256, at most 32).  One texel per atomic made every wave of
-// a 16-sample pass take a texel per 16 rays from the one counter, and the
-// pass ran at a third of the 256-sample pass's rays/s.
-__device__ inline uint32_t texel_batch(uint32_t S)
-{
-    return S >= 256 ? 1u : min(32u, 256u / max(S, 1u));
-}
 
 struct RayArgs {
     DevScene sc;
@@ -89,6 +82,7 @@ struct RayArgs {
     uint32_t S, P, t0, ntex, mwords;    // ntex texels, S rays each
     float mind, maxd;
     uint32_t refill;                    // refill a wave once this many lanes are idle
+    uint32_t tb;                        // TB kernels: texels taken from the counter at once (>= 1)
 };
 
 
@@ -368,11 +362,11 @@ constexpr int waves_for_stack()
     return waves > 8 ? 8 : (int)waves;
 }
 
-// TB: texels are taken from the launch's counter in batches (texel_batch),
-// for passes of fewer than 256 samples, with the wave's texel state in LDS;
-// otherwise one texel per atomic with the state in scalar registers, which
-// leaves the register allocation of the walk as it is (the paired AO + BN
-// pass at 256 samples ran 13 % slower with the batched hand-out's state).
+// TB: texels are taken from the launch's counter A.tb at a time (the pass's
+// hand-out tile, > 1), with the wave's texel state in LDS; otherwise one
+// texel per atomic with the state in scalar registers, which leaves the
+// register allocation of the walk as it is (the paired AO + BN pass at 256
+// samples ran 13 % slower with the batched hand-out's state).
 template <int KIND, bool COUNT, bool BATCH, int STK, bool TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for_stack<STK>()))) void k_rays(RayArgs A0,
                                                                                          const RayArgs* __restrict__ jobs,
@@ -448,7 +442,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 }
                 if (ws >= A.S) {
                     if (bq >= bend) {
-                        const uint32_t tb = texel_batch(A.S);
+                        const uint32_t tb = max(A.tb, 1u);  // a tile of 0 would never advance the counter
                         uint32_t nq = 0;
                         if (lane == 0) nq = atomicAdd(A.counter, tb);
                         nq = __builtin_amdgcn_readfirstlane(nq);  // wave-uniform, so the texel state stays scalar

@@ -140,6 +140,7 @@ struct fornos_solver {
     void* raybuf = nullptr;            // per-ray results of one launch (hit bytes or t)
     uint32_t chunk_texels = 0;         // texels per k_rays launch
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;  // k_rays launch shape
+    uint32_t tile = 0;                      // texels per hand-out (0: default_tile)
     int ray_order = FORNOS_RAY_ORDER_AUTO;  // order the texels' rays are handed out in
     uint32_t* counter = nullptr;       // k_rays work counter
     // per-launch events of the last timed run: (before k_rays, after k_rays, after k_reduce)
@@ -235,8 +236,20 @@ void launch_rays(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs&
         launch_rays_tb<KIND, COUNT, BATCH, false>(stack, blocks, st, A, jobs, nj);
 }
 
-// Passes of fewer samples than this take their texels in batches.
-constexpr uint32_t kTexelBatchBelow = 256;
+// A hemisphere pass's hand-out tile: texels a wave takes from the launch's
+// counter at once.  Default: about 256 rays' worth (1 at 256 samples and
+// more, at most 32); one texel per atomic made every wave of a 16-sample
+// pass take a texel per 16 rays from the one counter, and the pass ran at a
+// third of the 256-sample pass's rays/s.  A tile above 1 runs the TB kernels.
+uint32_t default_tile(uint32_t S)
+{
+    return S >= 256 ? 1u : std::min(32u, 256u / std::max(S, 1u));
+}
+
+uint32_t tile_of(const fornos_solver* s)
+{
+    return s->tile ? s->tile : default_tile(s->p.sample_count);
+}
 
 bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
 {
@@ -288,6 +301,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
     A.refill = s->refill;
+    A.tb = tile_of(s);
     ReduceArgs R;
     R.sc = sc->d;
     R.coords = mm->coords;
@@ -330,7 +344,7 @@ fornos_status launch_hemi(fornos_solver* s, uint32_t t0, uint32_t t1, bool count
         if (count)
             launch_rays<KIND, true, false>(s->stack, blocks, sc->stream, A, nullptr, 0);
         else
-            launch_rays<KIND, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, S < kTexelBatchBelow);
+            launch_rays<KIND, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, A.tb > 1);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<KIND>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, R);
         if (ev) HIP_TRY(hipEventRecord(ev[2], sc->stream));
@@ -365,6 +379,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint
     A.mind = s->p.min_distance;
     A.maxd = s->p.max_distance;
     A.refill = s->refill;
+    A.tb = tile_of(s);
     ReduceArgs Ra;
     Ra.sc = sc->d;
     Ra.coords = mm->coords;
@@ -408,7 +423,7 @@ fornos_status launch_pair(fornos_solver* s, uint32_t t0, uint32_t t1, const uint
             for (int i = 0; i < 3; ++i) HIP_TRY(hipEventCreate(&ev[i]));
             HIP_TRY(hipEventRecord(ev[0], sc->stream));
         }
-        launch_rays<kAOBN, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, S < kTexelBatchBelow);
+        launch_rays<kAOBN, false, false>(s->stack, blocks, sc->stream, A, nullptr, 0, A.tb > 1);
         if (ev) HIP_TRY(hipEventRecord(ev[1], sc->stream));
         hipLaunchKernelGGL(k_reduce<kAO>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Ra);
         hipLaunchKernelGGL(k_reduce<kBN>, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, sc->stream, Rb);
@@ -1476,6 +1491,20 @@ fornos_status fornos_solver_set_launch(fornos_solver* s, uint32_t blocks_per_cu,
     return FORNOS_OK;
 }
 
+fornos_status fornos_solver_set_tile(fornos_solver* s, uint32_t texels)
+{
+    if (!s || texels > 32) return set_error(FORNOS_INVALID_ARG, "fornos_solver_set_tile: texels in [0, 32] (0: default)");
+    s->tile = texels;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_solver_get_tile(const fornos_solver* s, uint32_t* texels)
+{
+    if (!s || !texels) return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_tile: null argument");
+    *texels = tile_of(s);
+    return FORNOS_OK;
+}
+
 fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t* best_bpc, uint32_t* best_refill,
                                      uint32_t* best_stack, float* best_ms)
 {
@@ -1492,43 +1521,85 @@ fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t*
     const uint32_t* list = nullptr;
     if (fornos_status e = pass_order(s, &list)) return e;
     hipStream_t st = s->mm->sc->stream;
-    const uint32_t keep[3] = {s->blocks_per_cu, s->refill, s->stack};
-    uint32_t cur[3] = {keep[0], keep[1], keep[2]};  // blocks per CU, refill, stack
-    auto time_shape = [&](const uint32_t* shape, float* ms) -> fornos_status {
+    const uint32_t keep[4] = {s->blocks_per_cu, s->refill, s->stack, s->tile};
+    const int keep_order = s->ray_order;
+    uint32_t cur[4] = {keep[0], keep[1], keep[2], tile_of(s)};  // blocks per CU, refill, stack, tile
+    // time a launch shape over `count` texels of `lst` (null: texels [0, count))
+    auto time_shape = [&](const uint32_t* shape, const uint32_t* lst, uint32_t count, float* ms) -> fornos_status {
         s->blocks_per_cu = shape[0];
         s->refill = shape[1];
         s->stack = shape[2];
+        s->tile = shape[3];
         for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
             Timer tm(st);
             if (fornos_status e = tm.start()) return e;
-            const fornos_status e = s->partner ? launch_pair(s, 0, m, list) : solver_launch(s, 0, m, false, list);
+            const fornos_status e = s->partner ? launch_pair(s, 0, count, lst) : solver_launch(s, 0, count, false, lst);
             if (e) return e;
             if (fornos_status e2 = tm.stop(ms)) return e2;
         }
         return FORNOS_OK;
     };
-    // coordinate descent: stack depth, then blocks per CU, then refill threshold
-    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}};
+    // coordinate descent: stack depth, then blocks per CU, then refill
+    // threshold, then the hand-out tile (around its default: 1 or 2 texels
+    // at 256 samples and more)
+    const uint32_t t0 = default_tile(S);
+    std::vector<uint32_t> tiles = {std::max(1u, t0 / 2), t0, std::min(32u, 2 * t0)};
+    tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
+    const std::vector<uint32_t> cand[4] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}, tiles};
     float best = FLT_MAX;
     fornos_status err = FORNOS_OK;
-    for (int axis : {2, 0, 1}) {
-        uint32_t pick = cur[axis];
-        for (uint32_t v : cand[axis]) {
-            uint32_t shape[3] = {cur[0], cur[1], cur[2]};
-            shape[axis] = v;
-            float ms = 0;
-            if ((err = time_shape(shape, &ms))) break;
-            if (ms < best) {
-                best = ms;
-                pick = v;
+    if (m > 0) {
+        for (int axis : {2, 0, 1, 3}) {
+            uint32_t pick = cur[axis];
+            for (uint32_t v : cand[axis]) {
+                uint32_t shape[4] = {cur[0], cur[1], cur[2], cur[3]};
+                shape[axis] = v;
+                float ms = 0;
+                if ((err = time_shape(shape, list, m, &ms))) break;
+                if (ms < best) {
+                    best = ms;
+                    pick = v;
+                }
             }
+            if (err) break;
+            cur[axis] = pick;
         }
-        if (err) break;
-        cur[axis] = pick;
+    }
+    // the ray order (the one sort of the rays: none, or by the Morton code of
+    // their texels' hit positions once per mapping result), timed on the same
+    // texels in both orders -- every k-th texel of the set, so that each
+    // order's rays in flight spread as they do in the whole pass
+    int order = effective_order(s);
+    if (!err && m > 0) {
+        const uint32_t* mort = nullptr;
+        err = meshmap_order(s->mm, &mort);
+        std::vector<uint32_t> perm(n), lists;
+        if (!err && hipMemcpy(perm.data(), mort, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            err = set_error(FORNOS_HIP_ERROR, "fornos_solver_autotune: reading the texel order failed");
+        const uint32_t k = std::max<uint32_t>(1, n / m);
+        if (!err) {
+            for (uint32_t g = 0; g < n; g += k) lists.push_back(g);
+            const size_t cnt = lists.size();
+            for (uint32_t g : perm)
+                if (g % k == 0) lists.push_back(g);
+            uint32_t* dl = nullptr;
+            if (lists.size() != 2 * cnt) err = set_error(FORNOS_HIP_ERROR, "fornos_solver_autotune: texel order is not a permutation");
+            if (!err) err = dev_alloc(&dl, lists.size());
+            if (!err && hipMemcpy(dl, lists.data(), sizeof(uint32_t) * lists.size(), hipMemcpyHostToDevice) != hipSuccess)
+                err = set_error(FORNOS_HIP_ERROR, "fornos_solver_autotune: uploading the sample texels failed");
+            float ms_t = 0, ms_m = 0;
+            if (!err) err = time_shape(cur, dl, (uint32_t)cnt, &ms_t);
+            if (!err) err = time_shape(cur, dl + cnt, (uint32_t)cnt, &ms_m);
+            if (hipStreamSynchronize(st) != hipSuccess && !err) err = set_error(FORNOS_HIP_ERROR, "fornos_solver_autotune: stream failed");
+            dev_free(dl);
+            if (!err) order = ms_m < ms_t ? FORNOS_RAY_ORDER_MORTON : FORNOS_RAY_ORDER_TEXELS;
+        }
     }
     s->blocks_per_cu = err ? keep[0] : cur[0];
     s->refill = err ? keep[1] : cur[1];
     s->stack = err ? keep[2] : cur[2];
+    s->tile = err ? keep[3] : cur[3];
+    s->ray_order = err || m == 0 ? keep_order : order;
     if (err) return err;
     if (best_bpc) *best_bpc = cur[0];
     if (best_refill) *best_refill = cur[1];
@@ -1581,7 +1652,7 @@ struct fornos_batch {
     uint32_t* counters = nullptr;
     int cus = 256;
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;
-    bool texel_batches = false;  // some job has fewer than kTexelBatchBelow samples
+    bool texel_batches = false;  // some job's hand-out tile is above 1 (TB kernel)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     ~fornos_batch()
@@ -1691,8 +1762,9 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         const uint32_t* list = nullptr;
         if (fornos_status st = pass_order(s, &list)) return st;
         A.list = list;
+        A.tb = tile_of(s);
+        if (A.tb > 1) b->texel_batches = true;
         b->jobs.push_back(A);
-        if (A.S < kTexelBatchBelow) b->texel_batches = true;
     }
     HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * n, hipMemcpyHostToDevice));
     *out = b.release();

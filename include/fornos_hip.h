@@ -351,14 +351,25 @@ fornos_status fornos_solver_set_launch(fornos_solver *s, uint32_t blocks_per_cu,
                                        uint32_t stack);
 fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_per_cu, uint32_t *refill,
                                        uint32_t *stack);
-/* Autotuner of a hemisphere solver's launch shape: times its pass over the
- * first `fraction` of its texels in the pass's order (at least 2^22 rays;
- * the paired AO + bent-normals traversal when paired) by coordinate descent
- * -- stack depth {16, 24, 32}, then blocks per CU {4, 6, 8, 12}, then the
- * refill threshold {16, 24, 32, 40, 48} -- and keeps the fastest.  Results
- * never depend on the shape; the sampled texels are left with their pass
- * results and the solver's progress is unchanged.  No reference
+/* Hand-out tile of a hemisphere solver's passes: texels a wave takes from
+ * the pass's counter at once, 1..32, or 0 (default) for 256 / sampleCount
+ * (1 at 256 samples and more).  A tile above 1 runs the ray kernel's variant
+ * that keeps a wave's texel state in LDS.  Results do not depend on it.
+ * get returns the tile in effect (the default resolved).  No reference
  * counterpart. */
+fornos_status fornos_solver_set_tile(fornos_solver *s, uint32_t texels);
+fornos_status fornos_solver_get_tile(const fornos_solver *s, uint32_t *texels);
+/* Autotuner of a hemisphere solver's launch: times its pass over the first
+ * `fraction` of its texels in the pass's order (at least 2^22 rays; the
+ * paired AO + bent-normals traversal when paired) by coordinate descent --
+ * stack depth {16, 24, 32}, then blocks per CU {4, 6, 8, 12}, then the
+ * refill threshold {16, 24, 32, 40, 48}, then the hand-out tile {half, once,
+ * twice the default} -- and keeps the fastest; then times the ray order
+ * (texel order against Morton order: whether the rays are sorted, once per
+ * mapping result) on the same texels, every k-th of the set, and sets the
+ * faster one (FORNOS_RAY_ORDER_TEXELS or _MORTON).  Results never depend on
+ * any of these; the sampled texels are left with their pass results and the
+ * solver's progress is unchanged.  No reference counterpart. */
 fornos_status fornos_solver_autotune(fornos_solver *s, float fraction, uint32_t *blocks_per_cu, uint32_t *refill,
                                      uint32_t *stack, float *best_ms);
 /* Order in which a hemisphere solver's passes hand out their texels' rays.

@@ -173,11 +173,14 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     box = nodes[0, :6].view(np.float32).astype(np.float64)
     long_rays = p["maxDistance"] >= 0.25 * np.float32(np.linalg.norm(box[3:] - box[:3]))
     assert s.effective_ray_order() == (s.RAY_ORDER_MORTON if long_rays else s.RAY_ORDER_TEXELS)
+    assert s.get_tile() == (1 if p["sampleCount"] >= 256 else 256 // p["sampleCount"])
     s.run_all()
     ref = s.getResults().view(np.uint32).copy()
     live = int(np.sum(field.tidx != MISS))
     for k, shape in enumerate([(8, 16, 24), (8, 16, 32), (1, 1, 16), (3, 64, 24), (16, 7, 32)]):
         s.set_launch(*shape)
+        s.set_tile((2, 1, 32, 0, 5)[k])  # texels per hand-out; 0: the default
+        assert s.get_tile() == ((2, 1, 32, 0, 5)[k] or (1 if p["sampleCount"] >= 256 else 256 // p["sampleCount"]))
         s.set_ray_order((s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON, s.RAY_ORDER_AUTO)[k % 3])
         assert s.effective_ray_order() in (s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON)
         if k % 3 < 2:
@@ -195,6 +198,8 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
         s.set_launch(8, 16, 20)
     with pytest.raises(fa.FornosError):
         s.set_ray_order(3)
+    with pytest.raises(fa.FornosError):
+        s.set_tile(33)
 
 
 @pytest.mark.parametrize("maxd", [0.05, 0.3])
@@ -302,9 +307,14 @@ def test_autotune_picks_a_shape_and_keeps_results(field, kind):
         s.pair(bn)
         with pytest.raises(fa.FornosError):
             bn.autotune(0.1)  # tuned through its AO solver
+    assert s.get_tile() == 8  # 256 / 32 samples
     bpc, refill, stack, ms = s.autotune(0.1)
     assert s.get_launch() == (bpc, refill, stack)
     assert bpc in (4, 6, 8, 12) and refill in (16, 24, 32, 40, 48) and stack in (16, 24, 32) and ms > 0
+    # the hand-out tile around its default, and an explicit ray order
+    assert s.get_tile() in (4, 8, 16)
+    assert s.get_ray_order() in (s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON)
+    assert s.effective_ray_order() == s.get_ray_order()
     s.reset()
     s.run_all()
     assert same_bits(s.getResults(), ref.getResults())

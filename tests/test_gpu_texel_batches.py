@@ -65,9 +65,13 @@ def test_small_sample_paired_pass_matches_separate_passes(holes):
 
 
 def test_batch_kernel_with_small_sample_jobs(holes):
-    # the batch kernel hands out texel batches when a job has fewer than 256
-    # samples; jobs of 16, 256 and 48 samples in one launch equal their passes
+    # the batch kernel hands out texel batches when a job's tile is above 1
+    # (each job keeps its solver's tile); jobs of 16, 256 (tile 4) and 48
+    # samples in one launch equal their passes.  Each job's tile must reach
+    # the job table: a tile of 0 there once kept the kernel from advancing
+    # its counter.
     jobs = [holes.solver("ao", sampleCount=S, minDistance=0.001, maxDistance=0.3) for S in (16, 256, 48)]
+    jobs[1].set_tile(4)
     refs = []
     for j in jobs:
         j.run_all()
