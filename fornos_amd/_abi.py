@@ -161,6 +161,9 @@ _SIGS = [
     ("fornos_batch_set_launch", None, [H, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("fornos_batch_get_launch", None, [H, u32p, u32p, u32p]),
     ("fornos_batch_autotune", None, [H, C.c_float, u32p, u32p, u32p, C.POINTER(C.c_float)]),
+    ("fornos_batch_get_tile", None, [H, H, u32p]),
+    ("fornos_batch_set_tile_shift", None, [H, C.c_int]),
+    ("fornos_batch_get_tile_shift", None, [H, C.POINTER(C.c_int)]),
     ("fornos_batch_run", None, [H, C.POINTER(C.c_float)]),
     ("fornos_batch_destroy", "void", [H]),
     # denoiser (include/fornos_denoise.h)

@@ -549,12 +549,29 @@ class Batch(_Handle):
         return a.value, b.value, c.value
 
     def autotune(self, fraction=0.05):
-        """Times launch shapes on a fraction of each job's texels, keeps the
-        fastest; returns (blocks_per_cu, refill, stack, ms)."""
+        """Times launch shapes and the jobs' hand-out tiles (halved, kept,
+        doubled) on a fraction of each job's texels, keeps the fastest;
+        returns (blocks_per_cu, refill, stack, ms)."""
         a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().fornos_batch_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(c),
                                           C.byref(ms)))
         return a.value, b.value, c.value, ms.value
+
+    def set_tile_shift(self, shift):
+        """Jobs' hand-out tiles: their solvers' halved (-1), kept (0) or
+        doubled (1).  Results do not depend on it."""
+        check(lib().fornos_batch_set_tile_shift(self._h, int(shift)))
+
+    def get_tile_shift(self):
+        t = C.c_int()
+        check(lib().fornos_batch_get_tile_shift(self._h, C.byref(t)))
+        return t.value
+
+    def get_tile(self, solver):
+        """The hand-out tile a job (one of this batch's solvers) runs with."""
+        t = C.c_uint32()
+        check(lib().fornos_batch_get_tile(self._h, solver.handle, C.byref(t)))
+        return t.value
 
     def run(self):
         ms = C.c_float()

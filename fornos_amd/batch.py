@@ -107,8 +107,8 @@ def _cache_key(jobs):
 
 
 def tune(batch, jobs, cache_path=None, fraction=0.05):
-    """Launch shape for `batch`: from the cache when present, else autotuned
-    (and written back)."""
+    """Launch shape and tile shift for `batch`: from the cache when present,
+    else autotuned (and written back)."""
     key = _cache_key(jobs)
     cache = {}
     if cache_path and os.path.exists(cache_path):
@@ -117,14 +117,18 @@ def tune(batch, jobs, cache_path=None, fraction=0.05):
     if key in cache:
         c = cache[key]
         batch.set_launch(c["blocks_per_cu"], c["refill"], c["stack"])
-        return {"blocks_per_cu": c["blocks_per_cu"], "refill": c["refill"], "stack": c["stack"], "cached": True}
+        batch.set_tile_shift(c.get("tile_shift", 0))
+        return {"blocks_per_cu": c["blocks_per_cu"], "refill": c["refill"], "stack": c["stack"],
+                "tile_shift": c.get("tile_shift", 0), "cached": True}
     bpc, refill, stack, ms = batch.autotune(fraction)
+    shift = batch.get_tile_shift()
     if cache_path:
-        cache[key] = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "sample_ms": ms}
+        cache[key] = {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "tile_shift": shift, "sample_ms": ms}
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
-    return {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "sample_ms": ms, "cached": False}
+    return {"blocks_per_cu": bpc, "refill": refill, "stack": stack, "tile_shift": shift, "sample_ms": ms,
+            "cached": False}
 
 
 def main(argv=None):

@@ -1652,7 +1652,8 @@ struct fornos_batch {
     uint32_t* counters = nullptr;
     int cus = 256;
     uint32_t blocks_per_cu = 8, refill = kRefill, stack = kStack;
-    bool texel_batches = false;  // some job's hand-out tile is above 1 (TB kernel)
+    std::vector<uint32_t> base_tile;  // per job: its solver's hand-out tile
+    int tile_shift = 0;               // the jobs' tiles are base_tile halved (-1), kept (0) or doubled (+1)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     ~fornos_batch()
@@ -1684,9 +1685,13 @@ fornos_status batch_enqueue(fornos_batch* b, const std::vector<RayArgs>& jobs, R
             hipLaunchKernelGGL(k_fill, dim3(div_up(J.ntex, kBlock)), dim3(kBlock), 0, st, J.cnt, (size_t)J.ntex, 0u);
     }
     uint64_t rays = 0;
-    for (const RayArgs& J : jobs) rays += (uint64_t)J.ntex * J.S;
+    bool tb = false;  // some job's hand-out tile is above 1: the TB kernel
+    for (const RayArgs& J : jobs) {
+        rays += (uint64_t)J.ntex * J.S;
+        tb |= J.tb > 1;
+    }
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)b->cus * b->blocks_per_cu, div_up(rays, kBlock));
-    if (blocks) launch_rays<kAO, false, true>(b->stack, blocks, st, jobs[0], djobs, nj, b->texel_batches);
+    if (blocks) launch_rays<kAO, false, true>(b->stack, blocks, st, jobs[0], djobs, nj, tb);
     for (uint32_t j = 0; j < nj; ++j) {
         const fornos_solver* s = b->solvers[j];
         ReduceArgs R{};
@@ -1710,6 +1715,12 @@ void batch_drop_graph(fornos_batch* b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+// A job's hand-out tile under the batch's tile shift.
+uint32_t shifted_tile(uint32_t base, int shift)
+{
+    return shift < 0 ? std::max(1u, base >> 1) : shift > 0 ? std::min(32u, base << 1) : base;
 }
 
 }  // namespace
@@ -1763,7 +1774,7 @@ fornos_status fornos_batch_create(fornos_solver* const* solvers, uint32_t n, for
         if (fornos_status st = pass_order(s, &list)) return st;
         A.list = list;
         A.tb = tile_of(s);
-        if (A.tb > 1) b->texel_batches = true;
+        b->base_tile.push_back(A.tb);
         b->jobs.push_back(A);
     }
     HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * n, hipMemcpyHostToDevice));
@@ -1835,11 +1846,15 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
     if (fornos_status st = dev_alloc(&dsample, sample.size())) return st;
     struct Free { RayArgs* p; ~Free() { dev_free(p); } } guard{dsample};
     const uint32_t keep[3] = {b->blocks_per_cu, b->refill, b->stack};
-    uint32_t cur[3] = {keep[0], keep[1], keep[2]};  // blocks per CU, refill, stack
+    // blocks per CU, refill, stack, tile shift + 1
+    uint32_t cur[4] = {keep[0], keep[1], keep[2], (uint32_t)(b->tile_shift + 1)};
     auto time_shape = [&](const uint32_t* shape, float* ms) -> fornos_status {
         b->blocks_per_cu = shape[0];
         b->stack = shape[2];
-        for (RayArgs& A : sample) A.refill = shape[1];
+        for (size_t j = 0; j < sample.size(); ++j) {
+            sample[j].refill = shape[1];
+            sample[j].tb = shifted_tile(b->base_tile[j], (int)shape[3] - 1);
+        }
         HIP_TRY(hipMemcpy(dsample, sample.data(), sizeof(RayArgs) * sample.size(), hipMemcpyHostToDevice));
         for (int rep = 0; rep < 2; ++rep) {  // the first run warms the caches
             Timer tm(b->stream);
@@ -1849,13 +1864,14 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
         }
         return FORNOS_OK;
     };
-    // coordinate descent: stack depth, then blocks per CU, then refill threshold
-    const std::vector<uint32_t> cand[3] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}};
+    // coordinate descent: stack depth, then blocks per CU, then refill
+    // threshold, then the jobs' hand-out tiles (halved, kept, doubled)
+    const std::vector<uint32_t> cand[4] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}, {0, 1, 2}};
     float best = FLT_MAX;
-    for (int axis : {2, 0, 1}) {
+    for (int axis : {2, 0, 1, 3}) {
         uint32_t pick = cur[axis];
         for (uint32_t v : cand[axis]) {
-            uint32_t shape[3] = {cur[0], cur[1], cur[2]};
+            uint32_t shape[4] = {cur[0], cur[1], cur[2], cur[3]};
             shape[axis] = v;
             float ms = 0;
             if (fornos_status st = time_shape(shape, &ms)) return st;
@@ -1869,6 +1885,9 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
     b->blocks_per_cu = keep[0];
     b->refill = keep[1];
     b->stack = keep[2];
+    b->tile_shift = (int)cur[3] - 1;
+    for (size_t j = 0; j < b->jobs.size(); ++j) b->jobs[j].tb = shifted_tile(b->base_tile[j], b->tile_shift);
+    HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * b->jobs.size(), hipMemcpyHostToDevice));
     if (fornos_status st = fornos_batch_set_launch(b, cur[0], cur[1], cur[2])) return st;
     batch_drop_graph(b);
     if (best_bpc) *best_bpc = cur[0];
@@ -1876,6 +1895,35 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
     if (best_stack) *best_stack = cur[2];
     if (best_ms) *best_ms = best;
     return FORNOS_OK;
+}
+
+fornos_status fornos_batch_set_tile_shift(fornos_batch* b, int shift)
+{
+    if (!b || shift < -1 || shift > 1) return set_error(FORNOS_INVALID_ARG, "fornos_batch_set_tile_shift: shift -1, 0 or 1");
+    if (shift == b->tile_shift) return FORNOS_OK;
+    b->tile_shift = shift;
+    for (size_t j = 0; j < b->jobs.size(); ++j) b->jobs[j].tb = shifted_tile(b->base_tile[j], shift);
+    HIP_TRY(hipMemcpy(b->djobs, b->jobs.data(), sizeof(RayArgs) * b->jobs.size(), hipMemcpyHostToDevice));
+    batch_drop_graph(b);
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_get_tile_shift(const fornos_batch* b, int* shift)
+{
+    if (!b || !shift) return set_error(FORNOS_INVALID_ARG, "fornos_batch_get_tile_shift: null argument");
+    *shift = b->tile_shift;
+    return FORNOS_OK;
+}
+
+fornos_status fornos_batch_get_tile(const fornos_batch* b, const fornos_solver* s, uint32_t* tile)
+{
+    if (!b || !s || !tile) return set_error(FORNOS_INVALID_ARG, "fornos_batch_get_tile: null argument");
+    for (size_t j = 0; j < b->solvers.size(); ++j)
+        if (b->solvers[j] == s) {
+            *tile = b->jobs[j].tb;
+            return FORNOS_OK;
+        }
+    return set_error(FORNOS_INVALID_ARG, "fornos_batch_get_tile: the solver is not a job of this batch");
 }
 
 void fornos_batch_destroy(fornos_batch* b) { delete b; }

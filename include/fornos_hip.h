@@ -413,9 +413,16 @@ fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_pe
                                       uint32_t *stack);
 /* Autotuner: times launch shapes on the first `fraction` of each job's
  * texels (at least 2^22 rays of each job) by coordinate descent (stack depth {16,24,32}, then blocks per CU
- * {4,6,8,12}, then refill {16,24,32,40,48}) and keeps the fastest. */
+ * {4,6,8,12}, then refill {16,24,32,40,48}, then the jobs' hand-out tiles: each job's solver tile
+ * halved, kept or doubled) and keeps the fastest.  Each job keeps its solver's ray order. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
                                     uint32_t *refill, uint32_t *stack, float *best_ms);
+/* The jobs' hand-out tiles: each job's solver tile halved (-1), kept (0,
+ * the default) or doubled (1), within 1..32; results do not depend on it. */
+fornos_status fornos_batch_set_tile_shift(fornos_batch *b, int shift);
+fornos_status fornos_batch_get_tile_shift(const fornos_batch *b, int *shift);
+/* The hand-out tile job `s` (one of the batch's solvers) runs with. */
+fornos_status fornos_batch_get_tile(const fornos_batch *b, const fornos_solver *s, uint32_t *tile);
 /* Runs every job once (graph launch); *ms = wall time of the graph on the
  * device (HIP events). */
 fornos_status fornos_batch_run(fornos_batch *b, float *ms);

@@ -4,6 +4,7 @@ identical bit for bit, on every graph replay and after autotuning."""
 import numpy as np
 import pytest
 
+import fornos_amd as fa
 from fornos_amd import batch
 
 pytestmark = pytest.mark.gpu
@@ -42,11 +43,20 @@ def test_batch_after_autotune_and_launch_changes(jobs, tmp_path):
     b = batch.Batch([j.solver for j in jobs])
     t = batch.tune(b, jobs, cache_path=str(tmp_path / "tune.json"), fraction=0.1)
     assert not t["cached"] and b.get_launch() == (t["blocks_per_cu"], t["refill"], t["stack"])
-    for shape in [None, (1, 1, 16), (3, 64, 32), (16, 8, 24), (2, 5, 16)]:
+    assert b.get_tile_shift() == t["tile_shift"] in (-1, 0, 1)
+    for k, shape in enumerate([None, (1, 1, 16), (3, 64, 32), (16, 8, 24), (2, 5, 16)]):
         if shape:
             b.set_launch(*shape)
+        b.set_tile_shift((0, -1, 1, 1, -1)[k])
+        base = [j.solver.get_tile() for j in jobs]
+        shifted = [max(1, t0 // 2) if b.get_tile_shift() < 0 else min(32, 2 * t0) if b.get_tile_shift() > 0 else t0
+                   for t0 in base]
+        assert [b.get_tile(j.solver) for j in jobs] == shifted
         b.run()
         for j, r in zip(jobs, seq):
             assert np.array_equal(j.solver.getResults().view(np.uint32), r), (shape, j.spec)
     b2 = batch.Batch([j.solver for j in jobs])
-    assert batch.tune(b2, jobs, cache_path=str(tmp_path / "tune.json"))["cached"]
+    c = batch.tune(b2, jobs, cache_path=str(tmp_path / "tune.json"))
+    assert c["cached"] and b2.get_tile_shift() == t["tile_shift"]
+    with pytest.raises(fa.FornosError):
+        b2.set_tile_shift(2)
