@@ -1494,6 +1494,8 @@ fornos_status fornos_solver_set_launch(fornos_solver* s, uint32_t blocks_per_cu,
 fornos_status fornos_solver_set_tile(fornos_solver* s, uint32_t texels)
 {
     if (!s || texels > 32) return set_error(FORNOS_INVALID_ARG, "fornos_solver_set_tile: texels in [0, 32] (0: default)");
+    if (!s->hemi())
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_set_tile: not a hemisphere solver (AO, bent normals, thickness)");
     s->tile = texels;
     return FORNOS_OK;
 }
@@ -1501,6 +1503,8 @@ fornos_status fornos_solver_set_tile(fornos_solver* s, uint32_t texels)
 fornos_status fornos_solver_get_tile(const fornos_solver* s, uint32_t* texels)
 {
     if (!s || !texels) return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_tile: null argument");
+    if (!s->hemi())
+        return set_error(FORNOS_INVALID_ARG, "fornos_solver_get_tile: not a hemisphere solver (AO, bent normals, thickness)");
     *texels = tile_of(s);
     return FORNOS_OK;
 }

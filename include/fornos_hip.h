@@ -355,8 +355,8 @@ fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_
  * the pass's counter at once, 1..32, or 0 (default) for 256 / sampleCount
  * (1 at 256 samples and more).  A tile above 1 runs the ray kernel's variant
  * that keeps a wave's texel state in LDS.  Results do not depend on it.
- * get returns the tile in effect (the default resolved).  No reference
- * counterpart. */
+ * get returns the tile in effect (the default resolved).  Other solvers:
+ * FORNOS_INVALID_ARG.  No reference counterpart. */
 fornos_status fornos_solver_set_tile(fornos_solver *s, uint32_t texels);
 fornos_status fornos_solver_get_tile(const fornos_solver *s, uint32_t *texels);
 /* Autotuner of a hemisphere solver's launch: times its pass over the first

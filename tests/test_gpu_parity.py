@@ -324,3 +324,7 @@ def test_autotune_picks_a_shape_and_keeps_results(field, kind):
         s.autotune(0.0)
     with pytest.raises(fa.FornosError):
         fa.HeightSolver(True, 0.0).init(tex, field.mm).autotune(0.1)
+    with pytest.raises(fa.FornosError):
+        fa.HeightSolver(True, 0.0).init(tex, field.mm).set_tile(2)
+    with pytest.raises(fa.FornosError):
+        fa.HeightSolver(True, 0.0).init(tex, field.mm).get_tile()
