@@ -19,7 +19,7 @@ import os
 import numpy as np
 
 from . import _abi
-from .api import AmbientOcclusionSolver, Batch, Mesh, MeshMapping, Scene, sample_table
+from .api import AmbientOcclusionSolver, Batch, Mesh, MeshMapping, Scene
 
 SIZES = (256, 512, 1024, 2048)
 SAMPLES = (32, 64, 128, 256)
@@ -87,8 +87,7 @@ class Job:
                                                     spec["cull"])
         self.map_ms = self.mapping.run_all()
         ao = spec["ao"]
-        self.solver = AmbientOcclusionSolver(ao["sampleCount"], ao["minDistance"], ao["maxDistance"],
-                                             table=sample_table(ao["sampleCount"])[0])
+        self.solver = AmbientOcclusionSolver(ao["sampleCount"], ao["minDistance"], ao["maxDistance"])
         self.solver.init(None, self.mapping)
         self.high_tris = hi.n_triangles
         self.low_tris = lo.n_triangles

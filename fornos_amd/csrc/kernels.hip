@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -557,6 +558,106 @@ fornos_status solver_launch(fornos_solver* s, uint32_t t0, uint32_t t1, bool cou
     }
     HIP_TRY(hipGetLastError());
     return FORNOS_OK;
+}
+
+// A hemisphere solver's two device copies of its sample table, as float4
+// rows: t4 in table order (w = 0) and r4 in ray order, each permutation's
+// samples sorted by the Morton code of their disk position (x, y) so that
+// consecutive rays of a texel -- the lanes of a wave -- point in similar
+// directions, with w carrying the sample's index, under which its result is
+// stored.  Both depend only on the table, so the last few small ones are
+// kept on the host (compared word for word, not by hash): the solvers of an
+// AO + bent-normal pair and of the next bake on the same table skip the
+// conversion and the sort.
+struct RayTable {
+    uint32_t S = 0, P = 0;
+    uint64_t hash = 0;         // of the table's words (solvers compare tables by it)
+    std::vector<float> table;  // 3 * P * S, as given
+    std::vector<float4> t4, r4;
+};
+
+std::shared_ptr<const RayTable> ray_table(const float* table, uint32_t S, uint32_t P)
+{
+    // at most 2 kept, of at most 2^20 samples each (44 bytes a sample)
+    constexpr size_t kKeep = 2, kMaxSamples = (size_t)1 << 20;
+    static std::mutex mu;
+    static std::vector<std::shared_ptr<const RayTable>> kept;  // most recent last
+    const size_t n = (size_t)P * S;
+    if (n <= kMaxSamples) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < kept.size(); ++i) {
+            const std::shared_ptr<const RayTable> k = kept[i];
+            if (k->S == S && k->P == P && std::memcmp(k->table.data(), table, sizeof(float) * 3 * n) == 0) {
+                kept.erase(kept.begin() + i);
+                kept.push_back(k);
+                return k;
+            }
+        }
+    }
+    auto rt = std::make_shared<RayTable>();
+    rt->S = S;
+    rt->P = P;
+    rt->t4.resize(n);
+    rt->r4.resize(n);
+    {
+        // FNV-1a over the table's 32-bit words in blocks of 2^16 words
+        // (hashed on several threads), then over the block hashes in order
+        const size_t words = 3 * n, block = 1u << 16;
+        std::vector<uint64_t> bh((words + block - 1) / block);
+        parallel_chunks(bh.size(), [&](size_t b0, size_t b1) {
+            for (size_t b = b0; b < b1; ++b) {
+                uint64_t h = 1469598103934665603ull;
+                for (size_t i = b * block; i < std::min(words, (b + 1) * block); ++i) {
+                    uint32_t w;
+                    std::memcpy(&w, &table[i], 4);
+                    h = (h ^ w) * 1099511628211ull;
+                }
+                bh[b] = h;
+            }
+        }, 1);
+        uint64_t h = 1469598103934665603ull;
+        for (uint64_t x : bh) h = (h ^ x) * 1099511628211ull;
+        rt->hash = h;
+    }
+    auto spread = [](uint32_t v) {  // 16 bits -> even bit positions
+        v &= 0xffffu;
+        v = (v | (v << 8)) & 0x00ff00ffu;
+        v = (v | (v << 4)) & 0x0f0f0f0fu;
+        v = (v | (v << 2)) & 0x33333333u;
+        v = (v | (v << 1)) & 0x55555555u;
+        return v;
+    };
+    auto quant = [](float x) {
+        const float u = std::min(1.0f, std::max(0.0f, 0.5f * (x + 1.0f)));
+        return (uint32_t)(u * 65535.0f);
+    };
+    // (key << 32 | index): unique, so an unstable sort orders them as a
+    // stable sort of the keys would; threads only where the work pays for
+    // starting them
+    parallel_chunks(P, [&](size_t p0, size_t p1) {
+        std::vector<uint64_t> key(S);
+        for (size_t p = p0; p < p1; ++p) {
+            float4* t4 = &rt->t4[p * S];
+            const float* row = table + 3 * p * S;
+            for (uint32_t i = 0; i < S; ++i) {
+                t4[i] = make_float4(row[3 * i], row[3 * i + 1], row[3 * i + 2], 0.0f);
+                key[i] = (uint64_t)(spread(quant(t4[i].x)) | (spread(quant(t4[i].y)) << 1)) << 32 | i;
+            }
+            std::sort(key.begin(), key.end());
+            for (uint32_t j = 0; j < S; ++j) {
+                const uint32_t i = (uint32_t)key[j];
+                const float4 v = t4[i];
+                rt->r4[p * S + j] = make_float4(v.x, v.y, v.z, __builtin_bit_cast(float, i));
+            }
+        }
+    }, std::max<size_t>(1, ((size_t)1 << 17) / std::max<uint32_t>(S, 1)));
+    if (n <= kMaxSamples) {
+        rt->table.assign(table, table + 3 * n);
+        std::lock_guard<std::mutex> lk(mu);
+        kept.push_back(rt);
+        if (kept.size() > kKeep) kept.erase(kept.begin());
+    }
+    return rt;
 }
 
 }  // namespace
@@ -1113,68 +1214,12 @@ fornos_status fornos_solver_create(int kind, const fornos_solver_params* params,
         if (!table || n_perm == 0) return set_error(FORNOS_INVALID_ARG, "fornos_solver_create: sample table missing");
         s->P = n_perm;
         s->work_total = (uint64_t)mm->work_count * S;
-        {
-            // FNV-1a over the table's 32-bit words in blocks of 2^16 words
-            // (hashed on several threads), then over the block hashes in order
-            const size_t words = 3 * (size_t)n_perm * S, block = 1u << 16;
-            std::vector<uint64_t> bh((words + block - 1) / block);
-            parallel_chunks(bh.size(), [&](size_t b0, size_t b1) {
-                for (size_t b = b0; b < b1; ++b) {
-                    uint64_t h = 1469598103934665603ull;
-                    for (size_t i = b * block; i < std::min(words, (b + 1) * block); ++i) {
-                        uint32_t w;
-                        std::memcpy(&w, &table[i], 4);
-                        h = (h ^ w) * 1099511628211ull;
-                    }
-                    bh[b] = h;
-                }
-            }, 1);
-            uint64_t h = 1469598103934665603ull;
-            for (uint64_t x : bh) h = (h ^ x) * 1099511628211ull;
-            s->table_hash = h;
-        }
-        std::vector<float4> t4((size_t)n_perm * S);
-        parallel_chunks(t4.size(), [&](size_t i0, size_t i1) {
-            for (size_t i = i0; i < i1; ++i) t4[i] = make_float4(table[3 * i], table[3 * i + 1], table[3 * i + 2], 0.0f);
-        }, 1u << 16);
-        if (fornos_status st = dev_alloc(&s->samples, t4.size())) return st;
-        HIP_TRY(hipMemcpy(s->samples, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));
-        // ray order: each permutation's samples sorted by the Morton code of
-        // their disk position (x, y), so that consecutive rays of a texel --
-        // the lanes of a wave -- point in similar directions.  w carries the
-        // sample's index, under which its result is stored.
-        {
-            std::vector<float4> r4(t4.size());
-            auto spread = [](uint32_t v) {  // 16 bits -> even bit positions
-                v &= 0xffffu;
-                v = (v | (v << 8)) & 0x00ff00ffu;
-                v = (v | (v << 4)) & 0x0f0f0f0fu;
-                v = (v | (v << 2)) & 0x33333333u;
-                v = (v | (v << 1)) & 0x55555555u;
-                return v;
-            };
-            auto quant = [](float x) {
-                const float u = std::min(1.0f, std::max(0.0f, 0.5f * (x + 1.0f)));
-                return (uint32_t)(u * 65535.0f);
-            };
-            parallel_chunks(n_perm, [&](size_t p0, size_t p1) {
-                std::vector<std::pair<uint32_t, uint32_t>> key(S);
-                for (size_t p = p0; p < p1; ++p) {
-                    const float4* row = &t4[p * S];
-                    for (uint32_t i = 0; i < S; ++i) key[i] = {spread(quant(row[i].x)) | (spread(quant(row[i].y)) << 1), i};
-                    std::stable_sort(key.begin(), key.end(),
-                                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-                                         return a.first < b.first;
-                                     });
-                    for (uint32_t j = 0; j < S; ++j) {
-                        const float4 v = row[key[j].second];
-                        r4[p * S + j] = make_float4(v.x, v.y, v.z, __builtin_bit_cast(float, key[j].second));
-                    }
-                }
-            });
-            if (fornos_status st = dev_alloc(&s->ray_samples, r4.size())) return st;
-            HIP_TRY(hipMemcpy(s->ray_samples, r4.data(), sizeof(float4) * r4.size(), hipMemcpyHostToDevice));
-        }
+        const std::shared_ptr<const RayTable> rt = ray_table(table, S, n_perm);
+        s->table_hash = rt->hash;
+        if (fornos_status st = dev_alloc(&s->samples, rt->t4.size())) return st;
+        HIP_TRY(hipMemcpy(s->samples, rt->t4.data(), sizeof(float4) * rt->t4.size(), hipMemcpyHostToDevice));
+        if (fornos_status st = dev_alloc(&s->ray_samples, rt->r4.size())) return st;
+        HIP_TRY(hipMemcpy(s->ray_samples, rt->r4.data(), sizeof(float4) * rt->r4.size(), hipMemcpyHostToDevice));
         if (fornos_status st = dev_alloc(&s->counts, mm->n)) return st;
         if (fornos_status st = dev_alloc(&s->stats, kStats)) return st;
         if (fornos_status st = dev_alloc(&s->counter, 1)) return st;
@@ -1943,12 +1988,18 @@ __device__ inline float finite_or0(float v) { return isfinite(v) ? v : 0.0f; }
 
 struct GatherArgs {
     const uint32_t* indices;
-    const float *ao, *bn, *pos, *nrm, *hgt;
+    const float *ao, *bn, *pos, *nrm, *hgt;  // pos, nrm, hgt null: from the mapping (below)
+    const float4 *coords, *spos, *snrm;      // the mapping's results and the scene's triangle soups
+    const uint32_t* tidx;
     float* image;
     uint32_t n, npix;
     float cx, cy, cz, inv_extent, dist_scale;
 };
 
+// MAPPED: position, object-space normal and mapping distance computed here
+// from the mapping, as the position, normals and height passes compute them
+// (texel_attr, texel_height), rather than read from those solvers' results
+template <bool MAPPED>
 __global__ void k_dn_gather(GatherArgs A)
 {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1956,15 +2007,26 @@ __global__ void k_dn_gather(GatherArgs A)
     const uint32_t pix = A.indices[q];
     if (pix >= A.npix) return;
     float* o = A.image + (size_t)pix * FORNOS_DN_IN;
-    o[0] = finite_or0(A.ao[q]);
-    for (int k = 0; k < 3; ++k) {
-        o[1 + k] = finite_or0(A.bn[3 * (size_t)q + k]);
-        o[7 + k] = finite_or0(A.nrm[3 * (size_t)q + k]);
+    V3 p, nr;
+    float h;
+    if (MAPPED) {
+        p = texel_attr<false>(A.coords, A.tidx, A.spos, q);
+        nr = texel_attr<true>(A.coords, A.tidx, A.snrm, q);
+        h = texel_height(A.coords, q);
+    } else {
+        p = mk(A.pos[3 * (size_t)q], A.pos[3 * (size_t)q + 1], A.pos[3 * (size_t)q + 2]);
+        nr = mk(A.nrm[3 * (size_t)q], A.nrm[3 * (size_t)q + 1], A.nrm[3 * (size_t)q + 2]);
+        h = A.hgt[q];
     }
-    o[4] = finite_or0((A.pos[3 * (size_t)q + 0] - A.cx) * A.inv_extent);
-    o[5] = finite_or0((A.pos[3 * (size_t)q + 1] - A.cy) * A.inv_extent);
-    o[6] = finite_or0((A.pos[3 * (size_t)q + 2] - A.cz) * A.inv_extent);
-    o[10] = finite_or0(A.hgt[q] * A.dist_scale);
+    o[0] = finite_or0(A.ao[q]);
+    for (int k = 0; k < 3; ++k) o[1 + k] = finite_or0(A.bn[3 * (size_t)q + k]);
+    o[4] = finite_or0((p.x - A.cx) * A.inv_extent);
+    o[5] = finite_or0((p.y - A.cy) * A.inv_extent);
+    o[6] = finite_or0((p.z - A.cz) * A.inv_extent);
+    o[7] = finite_or0(nr.x);
+    o[8] = finite_or0(nr.y);
+    o[9] = finite_or0(nr.z);
+    o[10] = finite_or0(h * A.dist_scale);
     o[FORNOS_DN_MASK_CHANNEL] = 1.0f;
 }
 
@@ -2057,16 +2119,32 @@ fornos_status fornos_denoise_gather(const fornos_meshmap* mm, const fornos_solve
     if (!mm->indices) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: the texel set has no map indices");
     if (fornos_status s = dn_solver_ok(ao, mm, FORNOS_SOLVER_AO, "ao")) return s;
     if (fornos_status s = dn_solver_ok(bn, mm, FORNOS_SOLVER_BENT_NORMALS, "bn")) return s;
-    if (fornos_status s = dn_solver_ok(position, mm, FORNOS_SOLVER_POSITION, "position")) return s;
-    if (fornos_status s = dn_solver_ok(normals, mm, FORNOS_SOLVER_NORMALS, "normals")) return s;
-    if (fornos_status s = dn_solver_ok(height, mm, FORNOS_SOLVER_HEIGHT, "height")) return s;
+    const bool mapped = !position && !normals && !height;
+    if (mapped) {
+        if (mm->offset < mm->work_count)
+            return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: the mesh mapping has not been run");
+    } else {
+        if (fornos_status s = dn_solver_ok(position, mm, FORNOS_SOLVER_POSITION, "position")) return s;
+        if (fornos_status s = dn_solver_ok(normals, mm, FORNOS_SOLVER_NORMALS, "normals")) return s;
+        if (fornos_status s = dn_solver_ok(height, mm, FORNOS_SOLVER_HEIGHT, "height")) return s;
+    }
     hipStream_t st = mm->sc->stream;
     const size_t npix = (size_t)mm->width * mm->height;
     hipLaunchKernelGGL(k_dn_fill, dim3(div_up(npix * FORNOS_DN_IN, kBlock)), dim3(kBlock), 0, st, image,
                        npix * FORNOS_DN_IN);
-    GatherArgs A{mm->indices, ao->out, bn->out, position->out, normals->out, height->out, image, mm->n,
-                 (uint32_t)npix, center[0], center[1], center[2], inv_extent, dist_scale};
-    if (mm->n) hipLaunchKernelGGL(k_dn_gather, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st, A);
+    GatherArgs A{mm->indices, ao->out, bn->out, nullptr, nullptr, nullptr, mm->coords, mm->sc->spos, mm->sc->snrm,
+                 mm->tidx, image, mm->n, (uint32_t)npix, center[0], center[1], center[2], inv_extent, dist_scale};
+    if (!mapped) {
+        A.pos = position->out;
+        A.nrm = normals->out;
+        A.hgt = height->out;
+    }
+    if (mm->n) {
+        if (mapped)
+            hipLaunchKernelGGL(k_dn_gather<true>, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st, A);
+        else
+            hipLaunchKernelGGL(k_dn_gather<false>, dim3(div_up(mm->n, kBlock)), dim3(kBlock), 0, st, A);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     return FORNOS_OK;

@@ -30,7 +30,7 @@ import numpy as np
 from . import _abi
 from ._abi import check, lib
 from .api import (AmbientOcclusionSolver, BentNormalsSolver, HeightSolver, Mesh, MeshMapping, NormalsSolver,
-                  PositionSolver, Scene, TexelArrays, _Handle, sample_table)
+                  PositionSolver, Scene, TexelArrays, _Handle)
 
 IN, OUT, MASK = 12, 4, 11
 LOW_S, HIGH_S = 16, 1024
@@ -178,10 +178,11 @@ class MeshJob:
 
     def solvers(self, mm, S, seed=1234):
         """AO and bent-normal solvers on the same rays, paired: one traversal
-        computes both (results identical to two passes)."""
-        table = sample_table(S, seed=seed)[0]
-        ao = AmbientOcclusionSolver(S, self.mind, self.maxd, table=table).init(None, mm)
-        bn = BentNormalsSolver(S, self.mind, self.maxd, tangentSpace=False, table=table).init(None, mm)
+        computes both (results identical to two passes).  The seeded table
+        is the process's shared copy (computed once per sample count and
+        seed)."""
+        ao = AmbientOcclusionSolver(S, self.mind, self.maxd, seed=seed).init(None, mm)
+        bn = BentNormalsSolver(S, self.mind, self.maxd, tangentSpace=False, seed=seed).init(None, mm)
         ao.pair(bn)
         return ao, bn
 
@@ -194,9 +195,12 @@ class MeshJob:
         return pos, nrm, hgt
 
     def gather(self, mm, ao, bn, g, image):
-        pos, nrm, hgt = g
+        """Network input from the solvers; g: the (position, normals, height)
+        solvers of gbuffers(), or None to compute their values from the
+        mapping inside the gather (the same bits, no separate passes)."""
+        hs = (None, None, None) if g is None else tuple(s.handle for s in g)
         c = np.ascontiguousarray(self.center, np.float32)
-        check(lib().fornos_denoise_gather(mm.handle, ao.handle, bn.handle, pos.handle, nrm.handle, hgt.handle,
+        check(lib().fornos_denoise_gather(mm.handle, ao.handle, bn.handle, *hs,
                                           c.ctypes.data_as(C.c_void_p), 2.0 / self.diag, 20.0 / self.diag,
                                           _p(image)))
         return image
@@ -260,9 +264,8 @@ def denoised_bake(job, net, seed=11, tile=32, z=4.0, slack=0.004, fallback=True,
     ao, bn = job.solvers(mm, LOW_S, seed)
     ao.run_all()
     bn.run_all()
-    g = job.gbuffers(mm)
     inp = torch.empty((1, W, W, IN), dtype=torch.float32, device="cuda")
-    job.gather(mm, ao, bn, g, inp)
+    job.gather(mm, ao, bn, None, inp)  # position, normal and distance from the mapping
     t1 = time.perf_counter()
     out, net_ms = net.forward(inp)
     t2 = time.perf_counter()

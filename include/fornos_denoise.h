@@ -65,8 +65,12 @@ void fornos_denoiser_destroy(fornos_denoiser *d);
 /* Network input image [height][width][FORNOS_DN_IN] (device memory, the map's
  * size) from the results of solvers on one mesh mapping whose texel set has
  * map indices: AO and bent normals (any sample count), position, normals
- * and height.  Positions are stored as (p - center) * inv_extent, the
- * mapping distance times dist_scale; texels outside the set are zero. */
+ * (object space) and height.  position, normals and height may all be null
+ * (not some of them): the values those passes would give are then computed
+ * from the mapping, which must have been run, within the gather, the same
+ * bits without the three passes.  Positions are stored as
+ * (p - center) * inv_extent, the mapping distance times dist_scale; texels
+ * outside the set are zero. */
 fornos_status fornos_denoise_gather(const fornos_meshmap *mm, const fornos_solver *ao, const fornos_solver *bn,
                                     const fornos_solver *position, const fornos_solver *normals,
                                     const fornos_solver *height, const float *center, float inv_extent,

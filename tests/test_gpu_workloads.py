@@ -10,6 +10,8 @@
   the check equals the plain 1024-sample bake bit for bit.
 - fornos_solver_run_texels, which that fallback uses: listed texels equal
   run_all's results.
+- the network input computed from the mapping inside the gather equals the
+  one gathered from the position, normals and height passes, bit for bit.
 """
 import numpy as np
 import pytest
@@ -144,6 +146,37 @@ def test_denoiser_gain_on_held_out_seed(net, seed):
     assert ae <= 0.01 and be <= 2.0
     assert ae * 3 < ae16 and be * 3 < be16  # what the network gains over its input
     assert t_dn * 5 < t_plain
+
+
+def test_gather_from_mapping_equals_gather_from_passes(gpu):
+    # the denoised bake gathers position, normal and distance from the
+    # mapping itself; the three passes give the same network input
+    import torch
+    from fornos_amd import denoise as dn
+    job = dn.MeshJob(batch.job_spec(2007), 512)
+    mm = job.mapping()
+    ao, bn = job.solvers(mm, dn.LOW_S, seed=11)
+    ao.run_all()
+    bn.run_all()
+    W = job.size
+    a = torch.full((1, W, W, dn.IN), 7.0, dtype=torch.float32, device="cuda")
+    b = torch.full((1, W, W, dn.IN), -7.0, dtype=torch.float32, device="cuda")
+    job.gather(mm, ao, bn, job.gbuffers(mm), a)
+    job.gather(mm, ao, bn, None, b)
+    ai, bi = a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32)
+    assert np.array_equal(ai, bi)
+    _, tidx = mm.read()
+    assert 0 < int(np.sum(tidx != MISS)) < W * W  # hits, misses and pixels outside the texel set
+    # all three passes or none; and the mapping must have been run
+    pos = fa.PositionSolver().init(None, mm)
+    c = np.zeros(3, np.float32)
+    with pytest.raises(fa.FornosError, match="normals must be a solver"):
+        fa._abi.check(fa.lib().fornos_denoise_gather(mm.handle, ao.handle, bn.handle, pos.handle, None, None,
+                                                     c.ctypes.data, 1.0, 1.0, dn._p(a)))
+    fresh = fa.MeshMapping().init_from_mesh(job.lo, job.dirs, W, W, job.scene, job.cull)
+    ao2, bn2 = job.solvers(fresh, dn.LOW_S, seed=11)
+    with pytest.raises(fa.FornosError, match="has not been run"):
+        job.gather(fresh, ao2, bn2, None, b)
 
 
 def test_denoiser_fallback_blocks_equal_plain_bake(net):
