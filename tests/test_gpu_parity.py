@@ -159,6 +159,26 @@ def test_ao_sample_counts(field, S):
     assert np.array_equal(cnt[:128], oc) and same_bits(res[:128], out)
 
 
+def test_solvers_on_tables_that_differ_in_one_sample_each(field):
+    # the library keeps the ray-ordered copies of recent sample tables; two
+    # tables that differ only in the last sample of every permutation (that
+    # sample mirrored through the disk's centre) each get their own
+    S = 32
+    t1 = fa.sample_table(S, seed=5)[0]
+    t2 = t1.copy()
+    last = t2.reshape(-1, S, 3)[:, -1]
+    last[:, :2] *= -1.0
+    a = field.solver("ao", sampleCount=S, table=t1)
+    b = field.solver("ao", sampleCount=S, table=t2)
+    c = field.solver("ao", sampleCount=S, table=t1)
+    for s in (a, b, c):
+        s.run_all()
+    assert np.array_equal(a.counts(), c.counts()) and same_bits(a.getResults(), c.getResults())
+    assert not np.array_equal(a.counts(), b.counts())
+    n = len(field.tidx)
+    assert field.check_texels("ao", b, np.r_[0:1024, n // 2:n // 2 + 1024]) == 2048
+
+
 @pytest.mark.parametrize("kind", ["ao", "bentnormals", "thickness"])
 def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     # the stack depth only decides which rays are redone on the stackless
