@@ -186,30 +186,53 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(conv_w
         static_assert(IH * IW * 2 + NB * KW <= NLDS, "the first layer's staging fits");
         uint4* const f_in = s_lds;
         uint4* const f_wt = s_lds + IH * IW * 2;
-        for (int i = tid; i < IH * IW * 2; i += kThreads) {
+        // every load of the thread is issued before the first is converted
+        // and stored (one memory latency per block, not one per load)
+        constexpr int NPX = IH * IW * 2, KPX = (NPX + kThreads - 1) / kThreads;
+        constexpr int NFW = NB * KW, KFW = (NFW + kThreads - 1) / kThreads;
+        float4 fv[KPX][2];
+        uint4 fw[KFW];
+#pragma unroll
+        for (int k = 0; k < KPX; ++k) {
+            const int i = tid + k * kThreads;
             const int h = i & 1, pix = i >> 1;
             const int iy = pix / IW, ix = pix - iy * IW;
             const int y = y0 + iy - 1, x = x0 + ix - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < H && x >= 0 && x < W) {
+            fv[k][0] = fv[k][1] = make_float4(0, 0, 0, 0);
+            if (i < NPX && y >= 0 && y < H && x >= 0 && x < W) {
                 const float* px = A.in_f32 + (((size_t)img * H + y) * W + x) * A.in_c;
-                uint16_t* v8 = reinterpret_cast<uint16_t*>(&v);
 #pragma unroll
                 for (int g = 0; g < 2; ++g) {
                     const uint32_t c = (uint32_t)(8 * h + 4 * g);
-                    if (c < A.in_c) {
-                        const float4 f = *reinterpret_cast<const float4*>(px + c);
-                        v8[4 * g] = f2bf(f.x);
-                        v8[4 * g + 1] = f2bf(f.y);
-                        v8[4 * g + 2] = f2bf(f.z);
-                        v8[4 * g + 3] = f2bf(f.w);
-                    }
+                    if (c < A.in_c) fv[k][g] = *reinterpret_cast<const float4*>(px + c);
                 }
             }
-            f_in[i] = v;
         }
-        for (int i = tid; i < NB * KW; i += kThreads)
-            f_wt[i] = *reinterpret_cast<const uint4*>(A.wts + (size_t)(cbase + i / KW) * (8 * KW) + 8 * (i % KW));
+#pragma unroll
+        for (int k = 0; k < KFW; ++k) {
+            const int i = tid + k * kThreads;
+            fw[k] = i < NFW ? *reinterpret_cast<const uint4*>(A.wts + (size_t)(cbase + i / KW) * (8 * KW) + 8 * (i % KW))
+                            : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < KPX; ++k) {
+            const int i = tid + k * kThreads;
+            uint4 v;
+            uint16_t* v8 = reinterpret_cast<uint16_t*>(&v);
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                v8[4 * g] = f2bf(fv[k][g].x);  // zero (outside the image, or past in_c) stays zero
+                v8[4 * g + 1] = f2bf(fv[k][g].y);
+                v8[4 * g + 2] = f2bf(fv[k][g].z);
+                v8[4 * g + 3] = f2bf(fv[k][g].w);
+            }
+            if (i < NPX) f_in[i] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < KFW; ++k) {
+            const int i = tid + k * kThreads;
+            if (i < NFW) f_wt[i] = fw[k];
+        }
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < 5; ++ks) {

@@ -29,8 +29,11 @@ def p(t):
     return C.c_void_p(t.data_ptr())
 
 
+# upsampled inputs through the multi-slice kernels (maps of whole and partial
+# 8 x 32 tiles) and through the one-slice one
 @pytest.mark.parametrize("up,c1,c2,cout,relu,hw", [(0, 32, 0, 32, 1, (16, 24)), (0, 64, 0, 64, 0, (8, 8)),
                                                   (1, 64, 32, 32, 1, (16, 16)), (1, 128, 64, 64, 1, (8, 16)),
+                                                  (1, 64, 32, 64, 0, (12, 40)), (1, 32, 0, 32, 1, (8, 64)),
                                                   (0, 32, 0, 128, 1, (6, 10))])
 def test_conv3x3_matches_oracle(torch, up, c1, c2, cout, relu, hw):
     from fornos_amd._abi import check, lib
