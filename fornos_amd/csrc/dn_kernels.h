@@ -141,6 +141,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(conv_w
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0, 0, 0, 0};
+    // the lane's output channels' biases, loaded with the first slice (not
+    // one load and wait each in the epilogue)
+    float bias_r[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bias_r[j] = A.bias ? A.bias[cbase + j * 16 + r] : 0.0f;
+    // the fused head's inputs likewise: this thread's pixel's residual
+    // channels and one of the head's 132 weights and biases (to LDS in the
+    // epilogue, once the slices' LDS reads are done)
+    const bool head = (FUSE & kFuseHead) && NB == 32 && A.head_out;
+    float4 hin = make_float4(0, 0, 0, 0);
+    float hwv = 0.0f;
+    if (head) {
+        const int y = y0 + tid / TW, x = x0 + tid % TW;
+        if (y < H && x < W) hin = *reinterpret_cast<const float4*>(A.head_in + (((size_t)img * H + y) * W + x) * A.head_cin);
+        if (tid < 128) hwv = A.head_w[tid];
+        else if (tid < 132) hwv = A.head_b[tid - 128];
+    }
 
     constexpr int NIN = IH * IW * 4, NWT = 9 * NB * 4;
     constexpr int KIN = (NIN + kThreads - 1) / kThreads, KWT = (NWT + kThreads - 1) / kThreads;
@@ -290,24 +307,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(conv_w
         // consecutive channels of a pixel (the LDS reads of the last slice are
         // done: the loop ends with a barrier)
         uint16_t* s_out = reinterpret_cast<uint16_t*>(s_lds);
-        // head: this thread's pixel's residual channels and the head's
-        // weights (to LDS past the outputs), loaded before the staging so
-        // their latency overlaps it
+        // head: its weights and biases go to LDS past the outputs
         float* s_head = reinterpret_cast<float*>(s_lds) + TH * TW * NB / 2;
-        float4 hin = make_float4(0, 0, 0, 0);
-        const bool head = (FUSE & kFuseHead) && NB == 32 && A.head_out;
         static_assert(!(FUSE & kFuseHead) || TH * TW == kThreads, "the head takes one pixel per thread");
         static_assert(TH * TW * NB * 2 + 132 * 4 <= NLDS * 16, "the head's weights fit past the outputs");
-        if (head) {
-            const int y = y0 + tid / TW, x = x0 + tid % TW;
-            if (y < H && x < W) hin = *reinterpret_cast<const float4*>(A.head_in + (((size_t)img * H + y) * W + x) * A.head_cin);
-            if (tid < 128) s_head[tid] = A.head_w[tid];
-            else if (tid < 132) s_head[tid] = A.head_b[tid - 128];
-        }
+        if (head && tid < 132) s_head[tid] = hwv;
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
             const int cl = j * 16 + r;
-            const float bias = A.bias ? A.bias[cbase + cl] : 0.0f;
+            const float bias = bias_r[j];
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
                 const int row = wave * (MT / 2) + (m >> 1);
@@ -381,7 +389,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(conv_w
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         const uint32_t co = cbase + j * 16 + r;
-        const float bias = A.bias ? A.bias[co] : 0.0f;
+        const float bias = bias_r[j];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int y = y0 + wave * (MT / 2) + (m >> 1);
