@@ -383,8 +383,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     constexpr bool kLdsState = TB && !BATCH;
     uint32_t wq = 0, ws = A.S, bq = 0, bend = 0;
     __shared__ uint32_t wstate[kBlock / 64][4];
+    // TB thickness: the mapping results (hit triangle or kMiss) of the
+    // wave's batch, loaded by its lanes at once when the batch is taken, at
+    // slot (texel & 31) (a batch is at most 32 consecutive texels), so taking
+    // a texel reads LDS rather than waiting on a load of its own (thickness-16
+    // on heightfield2048 15.9 -> 14.2 ms; AO, bent normals and the paired pass
+    // lost 0.5-1 % to the registers it takes, and load per texel)
+    constexpr bool kBatchLds = TB && KIND == kThick;
+    __shared__ uint32_t wbatch[kBatchLds ? kBlock / 64 : 1][32];
     const uint32_t lane = threadIdx.x & 63;
     uint32_t* const ws_of = wstate[threadIdx.x >> 6];
+    uint32_t* const bt_of = wbatch[kBatchLds ? threadIdx.x >> 6 : 0];
     if (kLdsState && lane == 0) {
         ws_of[0] = 0;
         ws_of[1] = 0xffffffffu;
@@ -442,7 +451,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                 }
                 if (ws >= A.S) {
                     if (bq >= bend) {
-                        const uint32_t tb = max(A.tb, 1u);  // a tile of 0 would never advance the counter
+                        // a tile of 0 would never advance the counter (the host
+                        // keeps tiles at most 32, which the batch's LDS slots need)
+                        const uint32_t tb = max(A.tb, 1u);
                         uint32_t nq = 0;
                         if (lane == 0) nq = atomicAdd(A.counter, tb);
                         nq = __builtin_amdgcn_readfirstlane(nq);  // wave-uniform, so the texel state stays scalar
@@ -452,9 +463,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
                         }
                         bq = nq;
                         bend = min(nq + tb, A.ntex);
+                        if (kBatchLds && lane < bend - bq)
+                            bt_of[(bq + lane) & 31] = A.tidx[texel_of(A.list, A.t0, bq + lane)];
                     }
                     wq = bq++;
-                    ws = __builtin_amdgcn_readfirstlane(A.tidx[texel_of(A.list, A.t0, wq)]) == kMiss ? A.S : 0;
+                    const uint32_t t = kBatchLds ? bt_of[wq & 31] : A.tidx[texel_of(A.list, A.t0, wq)];
+                    ws = __builtin_amdgcn_readfirstlane(t) == kMiss ? A.S : 0;
                     continue;
                 }
                 const uint32_t take = min(nidle - filled, A.S - ws);
