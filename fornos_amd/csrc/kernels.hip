@@ -241,15 +241,19 @@ void launch_rays(uint32_t stack, uint32_t blocks, hipStream_t st, const RayArgs&
 // counter at once.  Default: about 256 rays' worth (1 at 256 samples and
 // more, at most 32); one texel per atomic made every wave of a 16-sample
 // pass take a texel per 16 rays from the one counter, and the pass ran at a
-// third of the 256-sample pass's rays/s.  A tile above 1 runs the TB kernels.
-uint32_t default_tile(uint32_t S)
+// third of the 256-sample pass's rays/s.  Thickness takes at least 2: its
+// batched hand-out loads a batch's mapping results at once (thickness-256
+// on heightfield2048 185 -> 170 ms, on knot4096 4.42 -> 4.16 s).  A tile
+// above 1 runs the TB kernels.
+uint32_t default_tile(uint32_t S, int kind)
 {
-    return S >= 256 ? 1u : std::min(32u, 256u / std::max(S, 1u));
+    const uint32_t t = S >= 256 ? 1u : std::min(32u, 256u / std::max(S, 1u));
+    return kind == FORNOS_SOLVER_THICKNESS ? std::max(t, 2u) : t;
 }
 
 uint32_t tile_of(const fornos_solver* s)
 {
-    return s->tile ? s->tile : default_tile(s->p.sample_count);
+    return s->tile ? s->tile : default_tile(s->p.sample_count, s->kind);
 }
 
 bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
@@ -1590,8 +1594,8 @@ fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t*
     };
     // coordinate descent: stack depth, then blocks per CU, then refill
     // threshold, then the hand-out tile (around its default: 1 or 2 texels
-    // at 256 samples and more)
-    const uint32_t t0 = default_tile(S);
+    // at 256 samples and more; thickness 1, 2 or 4)
+    const uint32_t t0 = default_tile(S, s->kind);
     std::vector<uint32_t> tiles = {std::max(1u, t0 / 2), t0, std::min(32u, 2 * t0)};
     tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
     const std::vector<uint32_t> cand[4] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}, tiles};

@@ -353,7 +353,7 @@ fornos_status fornos_solver_get_launch(const fornos_solver *s, uint32_t *blocks_
                                        uint32_t *stack);
 /* Hand-out tile of a hemisphere solver's passes: texels a wave takes from
  * the pass's counter at once, 1..32, or 0 (default) for 256 / sampleCount
- * (1 at 256 samples and more).  A tile above 1 runs the ray kernel's variant
+ * (1 at 256 samples and more; thickness at least 2).  A tile above 1 runs the ray kernel's variant
  * that keeps a wave's texel state in LDS.  Results do not depend on it.
  * get returns the tile in effect (the default resolved).  Other solvers:
  * FORNOS_INVALID_ARG.  No reference counterpart. */

@@ -193,14 +193,15 @@ def test_launch_shape_and_stack_depth_do_not_change_results(field, kind):
     box = nodes[0, :6].view(np.float32).astype(np.float64)
     long_rays = p["maxDistance"] >= 0.25 * np.float32(np.linalg.norm(box[3:] - box[:3]))
     assert s.effective_ray_order() == (s.RAY_ORDER_MORTON if long_rays else s.RAY_ORDER_TEXELS)
-    assert s.get_tile() == (1 if p["sampleCount"] >= 256 else 256 // p["sampleCount"])
+    default_tile = max(1 if p["sampleCount"] >= 256 else 256 // p["sampleCount"], 2 if kind == "thickness" else 1)
+    assert s.get_tile() == default_tile
     s.run_all()
     ref = s.getResults().view(np.uint32).copy()
     live = int(np.sum(field.tidx != MISS))
     for k, shape in enumerate([(8, 16, 24), (8, 16, 32), (1, 1, 16), (3, 64, 24), (16, 7, 32)]):
         s.set_launch(*shape)
         s.set_tile((2, 1, 32, 0, 5)[k])  # texels per hand-out; 0: the default
-        assert s.get_tile() == ((2, 1, 32, 0, 5)[k] or (1 if p["sampleCount"] >= 256 else 256 // p["sampleCount"]))
+        assert s.get_tile() == ((2, 1, 32, 0, 5)[k] or default_tile)
         s.set_ray_order((s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON, s.RAY_ORDER_AUTO)[k % 3])
         assert s.effective_ray_order() in (s.RAY_ORDER_TEXELS, s.RAY_ORDER_MORTON)
         if k % 3 < 2:
