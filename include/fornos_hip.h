@@ -249,7 +249,10 @@ typedef struct fornos_solver fornos_solver;
  * (texel % n_perm) * sample_count + s (Shaders/ao_step1.comp:153); the
  * reference draws it with computeSamplesImportanceCosDir (Src/math.h:363-382,
  * n_perm = 4096); here the caller passes it as data.  Ignored (may be NULL)
- * for height/position/normals.  sample_count must be in [1, 4096]. */
+ * for height/position/normals.  sample_count must be in [1, 4096].  The
+ * library keeps host copies of the two most recent tables of at most 2^20
+ * directions with their ray order (44 bytes a direction, until the process
+ * ends): solvers created on an identical table reuse them. */
 fornos_status fornos_solver_create(int kind, const fornos_solver_params *params,
                                    fornos_meshmap *mm, const float *sample_table,
                                    uint32_t n_perm, fornos_solver **out);
