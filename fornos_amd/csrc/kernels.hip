@@ -2022,16 +2022,14 @@ __global__ void k_dn_gather(GatherArgs A)
         nr = mk(A.nrm[3 * (size_t)q], A.nrm[3 * (size_t)q + 1], A.nrm[3 * (size_t)q + 2]);
         h = A.hgt[q];
     }
-    o[0] = finite_or0(A.ao[q]);
-    for (int k = 0; k < 3; ++k) o[1 + k] = finite_or0(A.bn[3 * (size_t)q + k]);
-    o[4] = finite_or0((p.x - A.cx) * A.inv_extent);
-    o[5] = finite_or0((p.y - A.cy) * A.inv_extent);
-    o[6] = finite_or0((p.z - A.cz) * A.inv_extent);
-    o[7] = finite_or0(nr.x);
-    o[8] = finite_or0(nr.y);
-    o[9] = finite_or0(nr.z);
-    o[10] = finite_or0(h * A.dist_scale);
-    o[FORNOS_DN_MASK_CHANNEL] = 1.0f;
+    // the pixel's 12 channels as three 16-byte stores (48-byte records)
+    static_assert(FORNOS_DN_IN == 12 && FORNOS_DN_MASK_CHANNEL == 11, "the gather writes 12 channels, mask last");
+    float4* o4 = reinterpret_cast<float4*>(o);
+    o4[0] = make_float4(finite_or0(A.ao[q]), finite_or0(A.bn[3 * (size_t)q]), finite_or0(A.bn[3 * (size_t)q + 1]),
+                        finite_or0(A.bn[3 * (size_t)q + 2]));
+    o4[1] = make_float4(finite_or0((p.x - A.cx) * A.inv_extent), finite_or0((p.y - A.cy) * A.inv_extent),
+                        finite_or0((p.z - A.cz) * A.inv_extent), finite_or0(nr.x));
+    o4[2] = make_float4(finite_or0(nr.y), finite_or0(nr.z), finite_or0(h * A.dist_scale), 1.0f);
 }
 
 __global__ void k_dn_fill(float* p, size_t n)
@@ -2120,6 +2118,8 @@ fornos_status fornos_denoise_gather(const fornos_meshmap* mm, const fornos_solve
                                     float dist_scale, float* image)
 {
     if (!mm || !center || !image) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: null argument");
+    if (reinterpret_cast<uintptr_t>(image) % 16)
+        return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: image must be 16-byte aligned");
     if (!mm->indices) return set_error(FORNOS_INVALID_ARG, "fornos_denoise_gather: the texel set has no map indices");
     if (fornos_status s = dn_solver_ok(ao, mm, FORNOS_SOLVER_AO, "ao")) return s;
     if (fornos_status s = dn_solver_ok(bn, mm, FORNOS_SOLVER_BENT_NORMALS, "bn")) return s;

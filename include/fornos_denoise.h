@@ -70,7 +70,7 @@ void fornos_denoiser_destroy(fornos_denoiser *d);
  * from the mapping, which must have been run, within the gather, the same
  * bits without the three passes.  Positions are stored as
  * (p - center) * inv_extent, the mapping distance times dist_scale; texels
- * outside the set are zero. */
+ * outside the set are zero.  image must be 16-byte aligned. */
 fornos_status fornos_denoise_gather(const fornos_meshmap *mm, const fornos_solver *ao, const fornos_solver *bn,
                                     const fornos_solver *position, const fornos_solver *normals,
                                     const fornos_solver *height, const float *center, float inv_extent,

@@ -173,6 +173,10 @@ def test_gather_from_mapping_equals_gather_from_passes(gpu):
     with pytest.raises(fa.FornosError, match="normals must be a solver"):
         fa._abi.check(fa.lib().fornos_denoise_gather(mm.handle, ao.handle, bn.handle, pos.handle, None, None,
                                                      c.ctypes.data, 1.0, 1.0, dn._p(a)))
+    import ctypes as C
+    with pytest.raises(fa.FornosError, match="16-byte aligned"):  # checked before any launch
+        fa._abi.check(fa.lib().fornos_denoise_gather(mm.handle, ao.handle, bn.handle, None, None, None,
+                                                     c.ctypes.data, 1.0, 1.0, C.c_void_p(b.data_ptr() + 4)))
     fresh = fa.MeshMapping().init_from_mesh(job.lo, job.dirs, W, W, job.scene, job.cull)
     ao2, bn2 = job.solvers(fresh, dn.LOW_S, seed=11)
     with pytest.raises(fa.FornosError, match="has not been run"):
