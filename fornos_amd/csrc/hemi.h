@@ -651,6 +651,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
         float acc = 0;
         if (t != kMiss) {
             const float* tv = A.tval + (size_t)q * S;
+            // unrolled so several loads are in flight (the sums keep sample order)
+#pragma unroll 8
             for (uint32_t s = 0; s < S; ++s) {
                 const float v = tv[s];
                 if (v != FLT_MAX) ++cnt;
@@ -665,6 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs A)
             const float4* smp = A.samples + (size_t)(g % A.P) * S;
             const uint32_t* m = A.mask + (size_t)q * A.mwords;
             V3 acc = mk(0, 0, 0);
+#pragma unroll 8
             for (uint32_t s = 0; s < S; ++s) {
                 if ((m[s >> 5] >> (s & 31)) & 1u) {  // adds vec3(0): acc never holds -0
                     ++cnt;
