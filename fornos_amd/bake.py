@@ -49,8 +49,10 @@ def bake(cfg, out_dir=None, ext="png", dilation=export.DEFAULT_DILATION, prep=No
         solvers["ao"].pair(solvers["bentnormals"])  # one traversal for both maps
         report["paired"] = ["ao", "bentnormals"]
     # each map is exported on a host thread (its device part on the scene's
-    # export stream) while the next maps are baked
-    pool = ThreadPoolExecutor(max_workers=1) if out_dir else None
+    # export stream, one export at a time; the encodes overlap) while the
+    # next maps are baked: heightfield2048 with exports 0.44-0.49 s with one
+    # thread, 0.38-0.42 s with three
+    pool = ThreadPoolExecutor(max_workers=3) if out_dir else None
     if out_dir:
         os.makedirs(out_dir, exist_ok=True)
     exports = []
