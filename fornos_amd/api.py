@@ -439,7 +439,8 @@ class Solver(_Handle):
     def autotune(self, fraction=0.05):
         """Times launch shapes and hand-out tiles on the first `fraction` of
         the pass's texels, then the texel and Morton ray orders on the same
-        texels, and keeps the fastest; returns (blocks_per_cu, refill, stack,
+        texels, and keeps the fastest (a current setting gives way only to
+        one at least 1 % faster); returns (blocks_per_cu, refill, stack,
         ms) (fornos_solver_autotune; get_tile and get_ray_order give the
         rest).  Results never depend on any of these."""
         a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
@@ -550,8 +551,9 @@ class Batch(_Handle):
 
     def autotune(self, fraction=0.05):
         """Times launch shapes and the jobs' hand-out tiles (halved, kept,
-        doubled) on a fraction of each job's texels, keeps the fastest;
-        returns (blocks_per_cu, refill, stack, ms)."""
+        doubled) on a fraction of each job's texels, keeps the fastest (a
+        current setting gives way only to one at least 1 % faster); returns
+        (blocks_per_cu, refill, stack, ms)."""
         a, b, c, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().fornos_batch_autotune(self._h, float(fraction), C.byref(a), C.byref(b), C.byref(c),
                                           C.byref(ms)))

@@ -256,6 +256,9 @@ uint32_t tile_of(const fornos_solver* s)
     return s->tile ? s->tile : default_tile(s->p.sample_count, s->kind);
 }
 
+// The autotuners replace a current setting only by one at least 1 % faster.
+constexpr float kTuneMargin = 0.99f;
+
 bool launch_shape_ok(uint32_t blocks_per_cu, uint32_t refill, uint32_t stack)
 {
     return blocks_per_cu >= 1 && blocks_per_cu <= 16 && refill >= 1 && refill <= 64 &&
@@ -1599,23 +1602,32 @@ fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t*
     std::vector<uint32_t> tiles = {std::max(1u, t0 / 2), t0, std::min(32u, 2 * t0)};
     tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
     const std::vector<uint32_t> cand[4] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}, tiles};
+    // On each axis the current value is timed first and another replaces it
+    // only when at least kTuneMargin faster: the prefix's timings vary by
+    // about 1 %, and near-equal shapes chosen on noise ran the whole pass
+    // slower (heightfield2048 AO-256: refill 40 for 32, 185.4 -> 187.6 ms).
     float best = FLT_MAX;
     fornos_status err = FORNOS_OK;
     if (m > 0) {
         for (int axis : {2, 0, 1, 3}) {
             uint32_t pick = cur[axis];
+            float inc = 0;
+            if ((err = time_shape(cur, list, m, &inc))) break;
+            float axis_best = inc;
             for (uint32_t v : cand[axis]) {
+                if (v == cur[axis]) continue;
                 uint32_t shape[4] = {cur[0], cur[1], cur[2], cur[3]};
                 shape[axis] = v;
                 float ms = 0;
                 if ((err = time_shape(shape, list, m, &ms))) break;
-                if (ms < best) {
-                    best = ms;
+                if (ms < kTuneMargin * inc && ms < axis_best) {
+                    axis_best = ms;
                     pick = v;
                 }
             }
             if (err) break;
             cur[axis] = pick;
+            best = axis_best;
         }
     }
     // the ray order (the one sort of the rays: none, or by the Morton code of
@@ -1645,7 +1657,13 @@ fornos_status fornos_solver_autotune(fornos_solver* s, float fraction, uint32_t*
             if (!err) err = time_shape(cur, dl + cnt, (uint32_t)cnt, &ms_m);
             if (hipStreamSynchronize(st) != hipSuccess && !err) err = set_error(FORNOS_HIP_ERROR, "fornos_solver_autotune: stream failed");
             dev_free(dl);
-            if (!err) order = ms_m < ms_t ? FORNOS_RAY_ORDER_MORTON : FORNOS_RAY_ORDER_TEXELS;
+            // the automatic order is the incumbent, replaced only when the
+            // other is at least kTuneMargin faster
+            if (!err) {
+                const bool morton = order == FORNOS_RAY_ORDER_MORTON;
+                if (morton ? ms_t < kTuneMargin * ms_m : ms_m < kTuneMargin * ms_t)
+                    order = morton ? FORNOS_RAY_ORDER_TEXELS : FORNOS_RAY_ORDER_MORTON;
+            }
         }
     }
     s->blocks_per_cu = err ? keep[0] : cur[0];
@@ -1919,21 +1937,27 @@ fornos_status fornos_batch_autotune(fornos_batch* b, float fraction, uint32_t* b
     };
     // coordinate descent: stack depth, then blocks per CU, then refill
     // threshold, then the jobs' hand-out tiles (halved, kept, doubled)
+    // (the current value of each axis kept unless another is kTuneMargin faster)
     const std::vector<uint32_t> cand[4] = {{4, 6, 8, 12}, {16, 24, 32, 40, 48}, {16, 24, 32}, {0, 1, 2}};
     float best = FLT_MAX;
     for (int axis : {2, 0, 1, 3}) {
         uint32_t pick = cur[axis];
+        float inc = 0;
+        if (fornos_status st = time_shape(cur, &inc)) return st;
+        float axis_best = inc;
         for (uint32_t v : cand[axis]) {
+            if (v == cur[axis]) continue;
             uint32_t shape[4] = {cur[0], cur[1], cur[2], cur[3]};
             shape[axis] = v;
             float ms = 0;
             if (fornos_status st = time_shape(shape, &ms)) return st;
-            if (ms < best) {
-                best = ms;
+            if (ms < kTuneMargin * inc && ms < axis_best) {
+                axis_best = ms;
                 pick = v;
             }
         }
         cur[axis] = pick;
+        best = axis_best;
     }
     b->blocks_per_cu = keep[0];
     b->refill = keep[1];

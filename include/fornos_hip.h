@@ -367,10 +367,12 @@ fornos_status fornos_solver_get_tile(const fornos_solver *s, uint32_t *texels);
  * paired AO + bent-normals traversal when paired) by coordinate descent --
  * stack depth {16, 24, 32}, then blocks per CU {4, 6, 8, 12}, then the
  * refill threshold {16, 24, 32, 40, 48}, then the hand-out tile {half, once,
- * twice the default} -- and keeps the fastest; then times the ray order
- * (texel order against Morton order: whether the rays are sorted, once per
- * mapping result) on the same texels, every k-th of the set, and sets the
- * faster one (FORNOS_RAY_ORDER_TEXELS or _MORTON).  Results never depend on
+ * twice the default} -- keeping on each axis the current value unless
+ * another is at least 1 % faster; then times the ray order (texel order
+ * against Morton order: whether the rays are sorted, once per mapping
+ * result) on the same texels, every k-th of the set, and sets the other
+ * order than the automatic one only when it is at least 1 % faster
+ * (FORNOS_RAY_ORDER_TEXELS or _MORTON).  Results never depend on
  * any of these; the sampled texels are left with their pass results and the
  * solver's progress is unchanged.  No reference counterpart. */
 fornos_status fornos_solver_autotune(fornos_solver *s, float fraction, uint32_t *blocks_per_cu, uint32_t *refill,
@@ -417,7 +419,8 @@ fornos_status fornos_batch_get_launch(const fornos_batch *b, uint32_t *blocks_pe
 /* Autotuner: times launch shapes on the first `fraction` of each job's
  * texels (at least 2^22 rays of each job) by coordinate descent (stack depth {16,24,32}, then blocks per CU
  * {4,6,8,12}, then refill {16,24,32,40,48}, then the jobs' hand-out tiles: each job's solver tile
- * halved, kept or doubled) and keeps the fastest.  Each job keeps its solver's ray order. */
+ * halved, kept or doubled), keeping on each axis the current value unless another is at least
+ * 1 % faster.  Each job keeps its solver's ray order. */
 fornos_status fornos_batch_autotune(fornos_batch *b, float fraction, uint32_t *blocks_per_cu,
                                     uint32_t *refill, uint32_t *stack, float *best_ms);
 /* The jobs' hand-out tiles: each job's solver tile halved (-1), kept (0,
