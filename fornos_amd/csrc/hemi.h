@@ -387,8 +387,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_fo
     // wave's batch, loaded by its lanes at once when the batch is taken, at
     // slot (texel & 31) (a batch is at most 32 consecutive texels), so taking
     // a texel reads LDS rather than waiting on a load of its own (thickness-16
-    // on heightfield2048 15.9 -> 14.2 ms; AO, bent normals and the paired pass
-    // lost 0.5-1 % to the registers it takes, and load per texel)
+    // on heightfield2048 15.9 -> 14.2 ms).  AO, bent normals and the paired
+    // pass keep the load per texel: with the array they ran 0.5-1 % slower.
     constexpr bool kBatchLds = TB && KIND == kThick;
     __shared__ uint32_t wbatch[kBatchLds ? kBlock / 64 : 1][32];
     const uint32_t lane = threadIdx.x & 63;
